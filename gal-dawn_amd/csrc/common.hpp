@@ -1,0 +1,216 @@
+// Shared declarations of the HIP frontier engine: device graph, status
+// handling, wave64 helpers.  gfx950 only (wave64, 160 KB LDS per CU).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "../../include/dawn_capi.h"
+
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__gfx950__)
+#error "device code is written for gfx950 (CDNA4, wave64)"
+#endif
+
+namespace dawn {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;          // threads per workgroup for step kernels
+constexpr int kHeavyDegree = 32;     // rows >= this are expanded by a whole wave
+constexpr int32_t kInf = DAWN_INF;
+
+// Records the last HIP error text (per thread) for diagnostics.
+void set_hip_error(hipError_t e, const char* what, const char* file, int line);
+const char* last_error();
+
+#define DAWN_HIP(x)                                                  \
+  do {                                                               \
+    hipError_t e_ = (x);                                             \
+    if (e_ != hipSuccess) {                                          \
+      ::dawn::set_hip_error(e_, #x, __FILE__, __LINE__);             \
+      return e_ == hipErrorOutOfMemory ? DAWN_ERR_OUT_OF_MEMORY      \
+                                       : DAWN_ERR_HIP;               \
+    }                                                                \
+  } while (0)
+
+#define DAWN_TRY(x)           \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ != DAWN_OK) return rc_; \
+  } while (0)
+
+// Device allocation that frees itself.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  int alloc(size_t n) {
+    if (n <= count && p) return DAWN_OK;
+    release();
+    size_t bytes = sizeof(T) * (n ? n : 1);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_hip_error(e, "hipMalloc", __FILE__, __LINE__);
+      return e == hipErrorOutOfMemory ? DAWN_ERR_OUT_OF_MEMORY : DAWN_ERR_HIP;
+    }
+    count = n;
+    return DAWN_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+};
+
+struct MsWorkspace;   // 64-lane batched engine state (msbfs.hip)
+struct SsWorkspace;   // single-source engines state (bfs.hip / sssp.hip)
+
+}  // namespace dawn
+
+// Device-resident graph.  For undirected graphs the in-edge arrays alias the
+// out-edge arrays (the adjacency is symmetric).
+struct dawn_graph {
+  int32_t n = 0;
+  int64_t m = 0;
+  int32_t directed = 0;
+  int32_t weighted = 0;
+  int device = 0;
+  int32_t max_out_degree = 0;
+  int32_t max_in_degree = 0;
+  int32_t* row_ptr = nullptr;     // [n+1]
+  int32_t* col = nullptr;         // [m]
+  int32_t* weight = nullptr;      // [m] or null
+  int32_t* in_row_ptr = nullptr;  // CSC (directed only; else == row_ptr)
+  int32_t* in_col = nullptr;
+  int32_t* in_weight = nullptr;
+  std::vector<int32_t> h_row_ptr; // host copy of offsets (degrees for drivers)
+  hipStream_t stream = nullptr;
+  std::mutex lock;                // one traversal per graph at a time
+  dawn::MsWorkspace* ms = nullptr;
+  dawn::SsWorkspace* ss = nullptr;
+};
+
+namespace dawn {
+
+// ---- wave64 device helpers -------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t lanes_below_mask() {
+  int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Wave-aggregated append: lanes with pred get consecutive slots of *counter.
+// Must be reached by every lane that may append (ballot over active lanes).
+__device__ __forceinline__ int wave_append(unsigned int* counter, bool pred) {
+  uint64_t m = __ballot(pred);
+  if (m == 0) return -1;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned int base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
+  base = __shfl(base, leader);
+  return pred ? (int)(base + (unsigned int)__popcll(m & lanes_below_mask())) : -1;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    unsigned lo = __shfl_xor((unsigned)(x & 0xffffffffu), o);
+    unsigned hi = __shfl_xor((unsigned)(x >> 32), o);
+    x += ((unsigned long long)hi << 32) | lo;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_or_u64(uint64_t x) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    unsigned lo = __shfl_xor((unsigned)(x & 0xffffffffu), o);
+    unsigned hi = __shfl_xor((unsigned)(x >> 32), o);
+    x |= ((uint64_t)hi << 32) | lo;
+  }
+  return x;
+}
+
+__device__ __forceinline__ int wave_max_i32(int x) {
+  for (int o = 32; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o));
+  return x;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int o) {
+  unsigned lo = __shfl_xor((unsigned)(x & 0xffffffffu), o);
+  unsigned hi = __shfl_xor((unsigned)(x >> 32), o);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Transpose the 64x64 bit matrix held one row per lane: afterwards lane j
+// holds bit i = bit j of lane i's input.  Six butterfly rounds; round j
+// swaps bit j of the row index with bit j of the column index.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t x) {
+  const int l = lane_id();
+#pragma unroll
+  for (int j = 32, r = 0; j >= 1; j >>= 1, r++) {
+    const uint64_t M = (j == 32)  ? 0x00000000FFFFFFFFull
+                       : (j == 16) ? 0x0000FFFF0000FFFFull
+                       : (j == 8)  ? 0x00FF00FF00FF00FFull
+                       : (j == 4)  ? 0x0F0F0F0F0F0F0F0Full
+                       : (j == 2)  ? 0x3333333333333333ull
+                                   : 0x5555555555555555ull;
+    uint64_t o = shfl_xor_u64(x, j);
+    if ((l & j) == 0)
+      x = (x & M) | ((o & M) << j);
+    else
+      x = ((o & ~M) >> j) | (x & ~M);
+  }
+  return x;
+}
+
+// Exclusive scan of one int per thread over a kBlock-thread workgroup.
+// scratch: kBlock/kWave ints of LDS.  Returns the exclusive prefix; *total
+// receives the block sum.
+__device__ __forceinline__ int block_exclusive_scan(int x, int* scratch, int* total) {
+  const int l = lane_id();
+  const int w = threadIdx.x / kWave;
+  int inc = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int y = __shfl_up(inc, o);
+    if (l >= o) inc += y;
+  }
+  if (l == kWave - 1) scratch[w] = inc;
+  __syncthreads();
+  int base = 0, sum = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / kWave; i++) {
+    int s = scratch[i];
+    if (i < w) base += s;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return base + inc - x;
+}
+
+// Grid for grid-stride kernels: enough workgroups to fill 256 CUs several
+// times over, capped by the work available.
+inline unsigned grid_for(int64_t items, int per_block = kBlock, unsigned cap = 8192) {
+  int64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > (int64_t)cap) b = cap;
+  return (unsigned)b;
+}
+
+// Pointer kind of a caller buffer.
+bool is_device_pointer(const void* p);
+
+// Workspace release hooks (defined by the engines).
+void ms_workspace_free(MsWorkspace* ws);
+void ss_workspace_free(SsWorkspace* ws);
+
+}  // namespace dawn

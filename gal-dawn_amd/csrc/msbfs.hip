@@ -1,0 +1,759 @@
+// 64-source bit-packed unweighted DAWN (batched BOVM) on one HIP device.
+//
+// Replaces the reference's per-source loops DAWN::MSSP_GPU::run
+// (src/algorithm/gpu/mssp.cu:9-51), DAWN::APSP_GPU::run
+// (src/algorithm/gpu/apsp.cu:62-111) and DAWN::MSSP_CPU::run
+// (src/algorithm/cpu/mssp.cpp:9-32): one step advances 64 frontiers at
+// once, each vertex carrying one uint64 word with bit b = source lane b.
+//
+// Layout in HBM (per graph, reused across batches):
+//   seen   u64[npad]          lanes that have reached v
+//   next   u64[npad]          push/bitmap scatter target (kept all-zero
+//                             between steps by the finalize pass)
+//   hist   u64[H][npad]       level history: slot l%H holds the lanes that
+//                             reached v exactly at level l (= frontier of l)
+//   q      int32[2][n]        frontier vertex queue (any lane), level parity
+//   dist   int32[64][n]       output rows of the batch (caller or workspace)
+// Step modes (all produce the same next level):
+//   PULL    every vertex missing a lane ORs the previous level's words of its
+//           in-neighbours (CSC), early exit once all missing lanes are found;
+//           writes the level slot, seen and the queue directly.
+//   PUSH    queue of frontier vertices, workgroup-balanced expansion,
+//           64-bit atomicOr of (frontier & ~seen[v]) into next[v].
+//   BITMAP  dense scan of the previous level slot, same scatter as PUSH.
+//   PUSH/BITMAP are followed by a finalize pass that folds next into the
+//   level slot and seen.
+// After the last level one pass turns the history into int32 distances
+// (64 coalesced row stores per wave) and per-lane sums/counts (64x64 bit
+// transposes + popcount), so no distance is written twice.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+#include "sched.hpp"
+
+namespace dawn {
+
+constexpr int kLanes = 64;
+constexpr int kHist = 32;        // level slots
+constexpr int kChunk = 24;       // steps per captured chunk (< kHist)
+constexpr int kSlices = 6;       // bits of a relative level (2^6 > kHist)
+
+struct MsBatch {
+  int32_t nlanes;
+  int32_t lo;          // first level not yet flushed to dist
+  uint64_t lanemask;   // valid lanes
+  int32_t* dist;       // output rows: lane b at dist + b * stride
+  int64_t stride;
+  int32_t src[kLanes];
+};
+
+struct MsArgs {
+  int32_t n;
+  int32_t H;
+  int64_t npad;
+  int64_t m;
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* in_row_ptr;
+  const int32_t* in_col;
+  uint64_t* seen;
+  uint64_t* next;
+  uint64_t* hist;
+  int32_t* q;
+  LevelCounters* ring;
+  SchedState* st;
+  MsBatch* batch;
+};
+
+struct MsOut {
+  unsigned long long* sum;  // per lane, indexed [lane]
+  unsigned long long* reached;
+  unsigned long long* edges;
+  int32_t want_edges;
+};
+
+struct MsWorkspace {
+  int32_t n = 0;
+  int64_t npad = 0;
+  DevBuf<uint64_t> seen, next, hist, seen_bak;
+  DevBuf<int32_t> q, distbuf;
+  DevBuf<LevelCounters> ring, ring_bak;
+  DevBuf<SchedState> st;
+  DevBuf<MsBatch> batch;
+  DevBuf<float> trace_feat;
+  DevBuf<int32_t> trace_mode;
+  DevBuf<unsigned long long> lane_stats;  // 3 * 64
+  SchedState* h_state = nullptr;
+  MsBatch* h_batch = nullptr;
+  std::map<long long, hipGraphExec_t> graphs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, e_a = nullptr, e_b = nullptr;
+};
+
+void ms_workspace_free(MsWorkspace* ws) {
+  if (!ws) return;
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (ws->h_state) (void)hipHostFree(ws->h_state);
+  if (ws->h_batch) (void)hipHostFree(ws->h_batch);
+  for (hipEvent_t e : {ws->ev0, ws->ev1, ws->e_a, ws->e_b})
+    if (e) (void)hipEventDestroy(e);
+  delete ws;
+}
+
+namespace {
+
+__device__ __forceinline__ int out_deg(const MsArgs& a, int32_t v) {
+  return a.row_ptr[v + 1] - a.row_ptr[v];
+}
+
+__device__ __forceinline__ int owner_of(const int* off, int e) {
+  int lo = 0, hi = kBlock - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Per-thread counters of a level, flushed once per wave.
+struct MsAcc {
+  unsigned long long fv = 0, fe = 0, scout = 0, lanes = 0, uv = 0, ue = 0;
+  __device__ void flush(LevelCounters* c) {
+    unsigned long long a0 = wave_sum_u64(fv), a1 = wave_sum_u64(fe), a2 = wave_sum_u64(scout);
+    unsigned long long a3 = wave_sum_u64(lanes), a4 = wave_sum_u64(uv), a5 = wave_sum_u64(ue);
+    if (lane_id() == 0) {
+      if (a0) {
+        atomicAdd(&c->fv, a0);
+        atomicAdd(&c->fe, a1);
+        atomicAdd(&c->scout, a2);
+        atomicAdd(&c->flanes, a3);
+      }
+      if (a4) {
+        atomicAdd(&c->uv, a4);
+        atomicAdd(&c->ue, a5);
+      }
+    }
+  }
+};
+
+// Commits the new lanes nw of vertex v (v < n) for level lv: level slot,
+// seen, queue, counters.  Called by every lane of a wave (queue append is
+// wave-aggregated); lanes with v >= n pass valid = false.
+__device__ __forceinline__ void ms_commit(const MsArgs& a, bool valid, int32_t v,
+                                          uint64_t sv, uint64_t nw, uint64_t* slot_out,
+                                          int32_t* q_out, unsigned* qlen, MsAcc& acc) {
+  if (valid) {
+    slot_out[v] = nw;
+    if (nw) a.seen[v] = sv | nw;
+  }
+  const bool isf = valid && nw != 0;
+  int pos = wave_append(qlen, isf);
+  if (isf) {
+    q_out[pos] = v;
+    int d = out_deg(a, v);
+    acc.fv += 1;
+    acc.fe += (unsigned)d;
+    acc.scout += (unsigned)(d > 0 ? d : 1);
+    acc.lanes += (unsigned)__popcll(nw);
+  }
+  if (valid && (~(sv | nw)) != 0) {
+    acc.uv += 1;
+    acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+  }
+}
+
+// ---------------------------------------------------------------- kernels ---
+
+__global__ void ms_fill_kernel(MsArgs a, MsOut o) {
+  const uint64_t inval = ~a.batch->lanemask;
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.npad; v += gs) {
+    a.seen[v] = v < a.n ? inval : ~0ull;
+    a.hist[v] = 0ull;  // level-0 slot
+    a.next[v] = 0ull;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kRing) a.ring[threadIdx.x] = LevelCounters{};
+  if (blockIdx.x == 0 && threadIdx.x < kLanes) {
+    o.sum[threadIdx.x] = 0;
+    o.reached[threadIdx.x] = 0;
+    o.edges[threadIdx.x] = 0;
+  }
+}
+
+// Level 0: the batch's sources (duplicates share one frontier vertex).
+__global__ void ms_seed_kernel(MsArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const MsBatch& b = *a.batch;
+  LevelCounters c{};
+  for (int i = 0; i < b.nlanes; i++) {
+    const int32_t v = b.src[i];
+    const uint64_t bit = 1ull << i;
+    if (a.hist[v] == 0) {
+      a.q[c.qlen++] = v;
+      int d = out_deg(a, v);
+      c.fv += 1;
+      c.fe += (unsigned)d;
+      c.scout += (unsigned)d;
+    }
+    a.hist[v] |= bit;
+    a.seen[v] |= bit;
+    c.flanes += 1;
+  }
+  // Vertices still missing a lane after level 0, and their in-degree sum.
+  c.uv = (unsigned long long)a.n;
+  c.ue = (unsigned long long)a.m;
+  for (unsigned i = 0; i < c.qlen; i++) {
+    const int32_t v = a.q[i];
+    if (a.seen[v] == ~0ull) {
+      c.uv -= 1;
+      c.ue -= (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+    }
+  }
+  a.ring[0] = c;
+  SchedState st{};
+  st.step = 0;
+  st.e2c = a.m;
+  st.scout = (long long)c.scout;
+  st.force_mode = -1;
+  st.last_mode = DAWN_MODE_PUSH;
+  *a.st = st;
+}
+
+__global__ void ms_decide_kernel(MsArgs a, SchedParams p, SchedTrace tr) {
+  sched_decide(a.st, a.ring, p, tr);
+}
+
+__global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
+  __shared__ int s_off[kBlock];
+  __shared__ int s_beg[kBlock];
+  __shared__ uint64_t s_f[kBlock];
+  __shared__ int s_scan[kBlock / kWave];
+  const SchedState st = *a.st;
+  if (st.done) return;
+  const int lv = st.step;
+  const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad;
+  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad;
+  const int lane = lane_id();
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
+
+  if (st.mode == DAWN_MODE_PUSH) {
+    const int32_t* q_in = a.q + (int64_t)((lv - 1) & 1) * a.n;
+    const unsigned qn = a.ring[(lv - 1) % kRing].qlen;
+    for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
+      const int64_t i = c0 + threadIdx.x;
+      int beg = 0, deg = 0;
+      uint64_t f = 0;
+      if (i < qn) {
+        const int32_t u = q_in[i];
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+        f = prev[u];
+      }
+      int total;
+      const int off = block_exclusive_scan(deg, s_scan, &total);
+      s_off[threadIdx.x] = off;
+      s_beg[threadIdx.x] = beg;
+      s_f[threadIdx.x] = f;
+      __syncthreads();
+      for (int e = threadIdx.x; e < total; e += kBlock) {
+        const int j = owner_of(s_off, e);
+        const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
+        const uint64_t nv = s_f[j] & ~a.seen[v];
+        if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
+      }
+      __syncthreads();
+    }
+  } else if (st.mode == DAWN_MODE_BITMAP) {
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+         base += wstride) {
+      const int32_t u = (int32_t)(base + lane);
+      const uint64_t f = u < a.n ? prev[u] : 0ull;
+      const bool act = f != 0;
+      if (__ballot(act) == 0) continue;
+      int beg = 0, deg = 0;
+      if (act) {
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+      }
+      const bool light = act && deg < kHeavyDegree;
+      if (light) {
+        for (int j = 0; j < deg; j++) {
+          const int32_t v = a.col[beg + j];
+          const uint64_t nv = f & ~a.seen[v];
+          if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
+        }
+      }
+      uint64_t hm = __ballot(act && deg >= kHeavyDegree);
+      while (hm) {
+        const int l = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
+        const uint64_t hf = ((uint64_t)__shfl((unsigned)(f >> 32), l) << 32) |
+                            (unsigned)__shfl((unsigned)(f & 0xffffffffu), l);
+        for (int k = hb + lane; k < he; k += kWave) {
+          const int32_t v = a.col[k];
+          const uint64_t nv = hf & ~a.seen[v];
+          if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
+        }
+      }
+    }
+  } else {  // DAWN_MODE_PULL
+    int32_t* q_out = a.q + (int64_t)(lv & 1) * a.n;
+    LevelCounters* c_out = a.ring + (lv % kRing);
+    MsAcc acc;
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.npad;
+         base += wstride) {
+      const int32_t v = (int32_t)(base + lane);
+      const bool valid = v < a.n;
+      const uint64_t sv = valid ? a.seen[v] : ~0ull;
+      const uint64_t miss = ~sv;
+      int beg = 0, deg = 0;
+      if (miss) {
+        beg = a.in_row_ptr[v];
+        deg = a.in_row_ptr[v + 1] - beg;
+      }
+      uint64_t acc_w = 0;
+      const bool light = miss && deg < kHeavyDegree;
+      if (light) {
+        for (int j = 0; j < deg; j++) {
+          acc_w |= prev[a.in_col[beg + j]];
+          if ((acc_w & miss) == miss) break;
+        }
+      }
+      uint64_t hm = __ballot(miss && deg >= kHeavyDegree);
+      while (hm) {
+        const int l = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
+        const uint64_t hmiss = ((uint64_t)__shfl((unsigned)(miss >> 32), l) << 32) |
+                               (unsigned)__shfl((unsigned)(miss & 0xffffffffu), l);
+        uint64_t part = 0;
+        for (int k0 = hb; k0 < he; k0 += kWave) {
+          const int k = k0 + lane;
+          if (k < he) part |= prev[a.in_col[k]];
+          const uint64_t all = wave_or_u64(part);
+          if ((all & hmiss) == hmiss) break;
+        }
+        part = wave_or_u64(part);
+        if (lane == l) acc_w = part;
+      }
+      const uint64_t nw = acc_w & miss;
+      ms_commit(a, valid, v, sv, nw, cur, q_out, &c_out->qlen, acc);
+    }
+    acc.flush(c_out);
+  }
+}
+
+// Folds next[] into the level slot after PUSH / BITMAP.
+__global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
+  const SchedState st = *a.st;
+  if (st.done || st.mode == DAWN_MODE_PULL) return;
+  const int lv = st.step;
+  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad;
+  int32_t* q_out = a.q + (int64_t)(lv & 1) * a.n;
+  LevelCounters* c_out = a.ring + (lv % kRing);
+  MsAcc acc;
+  const int64_t wstride = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.npad;
+       base += wstride) {
+    const int32_t v = (int32_t)(base + lane_id());
+    const bool valid = v < a.n;
+    uint64_t sv = ~0ull, nw = 0;
+    if (valid) {
+      const uint64_t nx = a.next[v];
+      sv = a.seen[v];
+      if (nx) a.next[v] = 0ull;
+      nw = nx & ~sv;
+    }
+    ms_commit(a, valid, v, sv, nw, cur, q_out, &c_out->qlen, acc);
+  }
+  acc.flush(c_out);
+}
+
+// Turns levels [lo, hi) of the history into distances.  from_state: lo and
+// hi come from the batch / scheduler state and the pass runs only once the
+// traversal is done (then also writes INF for lanes that never arrived).
+__global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int from_state,
+                                                          int lo_arg, int hi_arg, int final_arg) {
+  __shared__ unsigned long long s_sum[kLanes], s_reach[kLanes], s_edges[kLanes];
+  int lo = lo_arg, hi = hi_arg, fin = final_arg;
+  if (from_state) {
+    const SchedState st = *a.st;
+    if (!st.done) return;
+    lo = a.batch->lo;
+    hi = st.levels;
+    fin = 1;
+  }
+  const int nl = a.batch->nlanes;
+  int32_t* const dist = a.batch->dist;
+  const int64_t stride = a.batch->stride;
+  if (threadIdx.x < kLanes) {
+    s_sum[threadIdx.x] = 0;
+    s_reach[threadIdx.x] = 0;
+    s_edges[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  // Levels are stored relative to lo in K bit slices: S_k = OR of the
+  // level words whose (l - lo) has bit k set.  hi - lo <= kHist.
+  int K = 0;
+  while ((1 << K) < hi - lo) K++;
+  const int lane = lane_id();
+  const int64_t wstride = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+       base += wstride) {
+    const int32_t v = (int32_t)(base + lane);
+    const bool valid = v < a.n;
+    uint64_t F = 0, S[kSlices];
+#pragma unroll
+    for (int k = 0; k < kSlices; k++) S[k] = 0;
+    uint64_t sv = 0;
+    if (valid) {
+      sv = a.seen[v];
+      for (int l = lo; l < hi; l++) {
+        const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad + v];
+        const int r = l - lo;
+        F |= w;
+#pragma unroll
+        for (int k = 0; k < kSlices; k++)
+          if ((r >> k) & 1) S[k] |= w;
+      }
+    }
+    if (dist && valid) {
+      int32_t* col0 = dist + v;
+      for (int b = 0; b < nl; b++) {
+        int32_t d = lo;
+#pragma unroll
+        for (int k = 0; k < kSlices; k++)
+          if (k < K) d += (int32_t)((S[k] >> b) & 1ull) << k;
+        if ((F >> b) & 1ull)
+          col0[(int64_t)b * stride] = d;
+        else if (fin && !((sv >> b) & 1ull))
+          col0[(int64_t)b * stride] = kInf;
+      }
+    }
+    // Per-lane sums: sum_b = lo * reach_b + sum_k 2^k * |{v : bit b of S_k}|.
+    const uint64_t tf = wave_transpose64(F);
+    const unsigned long long reach = (unsigned long long)__popcll(tf);
+    unsigned long long sum = reach * (unsigned long long)lo;
+#pragma unroll
+    for (int k = 0; k < kSlices; k++) {
+      if (k < K) {
+        const uint64_t t = wave_transpose64(S[k]);
+        sum += (unsigned long long)__popcll(t) << k;
+      }
+    }
+    unsigned long long edges = 0;
+    if (o.want_edges) {
+      const int d = valid ? out_deg(a, v) : 0;
+      for (int i = 0; i < kWave; i++) {
+        const int di = __shfl(d, i);
+        if ((tf >> i) & 1ull) edges += (unsigned)di;
+      }
+    }
+    if (sum) atomicAdd(&s_sum[lane], sum);
+    if (reach) atomicAdd(&s_reach[lane], reach);
+    if (edges) atomicAdd(&s_edges[lane], edges);
+  }
+  __syncthreads();
+  if (threadIdx.x < nl) {
+    const int b = threadIdx.x;
+    if (s_sum[b]) atomicAdd(&o.sum[b], s_sum[b]);
+    if (s_reach[b]) atomicAdd(&o.reached[b], s_reach[b]);
+    if (s_edges[b]) atomicAdd(&o.edges[b], s_edges[b]);
+  }
+}
+
+int ensure_ms(dawn_graph* g) {
+  if (!g->ms) g->ms = new MsWorkspace();
+  MsWorkspace* ws = g->ms;
+  if (ws->n == g->n && ws->seen.p) return DAWN_OK;
+  ws->n = g->n;
+  ws->npad = ((int64_t)g->n + 63) / 64 * 64;
+  DAWN_TRY(ws->seen.alloc(ws->npad));
+  DAWN_TRY(ws->next.alloc(ws->npad));
+  DAWN_TRY(ws->hist.alloc((size_t)kHist * ws->npad));
+  DAWN_TRY(ws->q.alloc((size_t)2 * std::max(g->n, 1)));
+  DAWN_TRY(ws->ring.alloc(kRing));
+  DAWN_TRY(ws->ring_bak.alloc(kRing));
+  DAWN_TRY(ws->st.alloc(1));
+  DAWN_TRY(ws->batch.alloc(1));
+  DAWN_TRY(ws->trace_feat.alloc((size_t)kTraceMax * 8));
+  DAWN_TRY(ws->trace_mode.alloc(kTraceMax));
+  DAWN_TRY(ws->lane_stats.alloc(3 * kLanes));
+  DAWN_HIP(hipMemset(ws->next.p, 0, sizeof(uint64_t) * ws->npad));
+  if (!ws->h_state) DAWN_HIP(hipHostMalloc((void**)&ws->h_state, sizeof(SchedState)));
+  if (!ws->h_batch) DAWN_HIP(hipHostMalloc((void**)&ws->h_batch, sizeof(MsBatch)));
+  for (hipEvent_t* e : {&ws->ev0, &ws->ev1, &ws->e_a, &ws->e_b})
+    if (!*e) DAWN_HIP(hipEventCreate(e));
+  return DAWN_OK;
+}
+
+MsArgs make_args(dawn_graph* g) {
+  MsWorkspace* ws = g->ms;
+  MsArgs a{};
+  a.n = g->n;
+  a.H = kHist;
+  a.npad = ws->npad;
+  a.m = g->m;
+  a.row_ptr = g->row_ptr;
+  a.col = g->col;
+  a.in_row_ptr = g->in_row_ptr;
+  a.in_col = g->in_col;
+  a.seen = ws->seen.p;
+  a.next = ws->next.p;
+  a.hist = ws->hist.p;
+  a.q = ws->q.p;
+  a.ring = ws->ring.p;
+  a.st = ws->st.p;
+  a.batch = ws->batch.p;
+  return a;
+}
+
+SchedParams make_params(dawn_graph* g, int32_t policy) {
+  SchedParams p{};
+  p.n = g->n;
+  p.m = g->m;
+  p.policy = policy;
+  p.alpha = 15;
+  p.beta = 18;
+  p.batched = 1;
+  p.allow_pull = 1;
+  p.avg_degree = g->n ? (float)((double)g->m / g->n) : 0.f;
+  p.max_degree = (float)g->max_out_degree;
+  p.mlp = sched_mlp_device(g->device);
+  p.mlp_norm = sched_norm_device(g->device);
+  return p;
+}
+
+unsigned ms_grid(const dawn_graph* g) { return grid_for(std::max<int64_t>(g->n, 1), kBlock, 8192); }
+
+// Stages one batch's sources (taken mod n) into the pinned batch block.
+void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
+  MsBatch& b = *ws->h_batch;
+  std::memset(&b, 0, sizeof b);
+  b.nlanes = lanes;
+  b.lo = 0;
+  b.lanemask = lanes >= 64 ? ~0ull : ((1ull << lanes) - 1);
+  for (int i = 0; i < lanes; i++) {
+    int64_t s = sources[i] % (int64_t)ws->n;
+    if (s < 0) s += ws->n;
+    b.src[i] = (int32_t)s;
+  }
+}
+
+}  // namespace
+
+// Runs the batches of an unweighted multi-source traversal.
+//   dist: caller buffer (S*n int32, host/device) or null
+//   sums/reached/edges: per source (host) or null
+// mode_trace (optional): receives per-step modes of every batch.
+int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
+              int64_t* sums, int64_t* reached, int64_t* edges, int32_t policy,
+              dawn_stats* stats) {
+  if (!g || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  DAWN_TRY(ensure_ms(g));
+  MsWorkspace* ws = g->ms;
+  hipStream_t s = g->stream;
+  const bool dist_dev = dist && is_device_pointer(dist);
+  if (!dist_dev) DAWN_TRY(ws->distbuf.alloc((size_t)kLanes * g->n));
+
+  MsArgs a = make_args(g);
+  SchedParams p = make_params(g, policy);
+  SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
+  const unsigned grid = ms_grid(g);
+  unsigned long long* ls = ws->lane_stats.p;
+
+  // Graph of one batch: fill, seed, kChunk steps, final pass (no-op unless
+  // the traversal finished within the chunk).  Sources and the output
+  // pointer live in the device batch block, so one graph serves all batches.
+  auto get_graph = [&](int want_edges, hipGraphExec_t* exec) -> int {
+    const long long key = ((long long)policy << 8) | want_edges;
+    auto it = ws->graphs.find(key);
+    if (it != ws->graphs.end()) {
+      *exec = it->second;
+      return DAWN_OK;
+    }
+    MsOut o{ls, ls + kLanes, ls + 2 * kLanes, want_edges};
+    hipGraph_t graph;
+    DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o);
+    ms_seed_kernel<<<1, 64, 0, s>>>(a);
+    for (int i = 0; i < kChunk; i++) {
+      ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+      ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+      ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
+    }
+    ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
+    DAWN_HIP(hipStreamEndCapture(s, &graph));
+    hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DAWN_HIP(e);
+    ws->graphs[key] = *exec;
+    return DAWN_OK;
+  };
+
+  const int want_edges = (stats || edges) ? 1 : 0;
+  std::vector<unsigned long long> h_ls(3 * kLanes);
+  std::vector<int32_t> h_modes(kTraceMax);
+  float total_ms = 0.f;
+  for (int32_t b0 = 0; b0 < S; b0 += kLanes) {
+    const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
+    int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : ws->distbuf.p;
+    hipGraphExec_t exec;
+    DAWN_TRY(get_graph(want_edges, &exec));
+    stage_batch(ws, sources + b0, lanes);
+    ws->h_batch->dist = out_base;
+    ws->h_batch->stride = g->n;
+    DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+    DAWN_HIP(hipEventRecord(ws->ev0, s));
+    DAWN_HIP(hipGraphLaunch(exec, s));
+    DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    // Long traversals: more chunks, flushing all but the newest level
+    // before the history ring wraps.
+    int lo = 0;
+    int64_t guard_steps = kChunk;
+    while (!ws->h_state->done) {
+      MsOut o{ls, ls + kLanes, ls + 2 * kLanes, want_edges};
+      const int stp = ws->h_state->step;
+      if (stp > lo) {
+        ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, stp, 0);
+        lo = stp;
+      }
+      for (int i = 0; i < kChunk; i++) {
+        ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+        ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+        ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
+      }
+      guard_steps += kChunk;
+      DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
+      DAWN_HIP(hipStreamSynchronize(s));
+      if (ws->h_state->done) {
+        ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels, 1);
+        break;
+      }
+      if (guard_steps > (int64_t)g->n + 2 * kChunk) return DAWN_ERR_HIP;
+    }
+    DAWN_HIP(hipGetLastError());
+    DAWN_HIP(hipEventRecord(ws->ev1, s));
+    DAWN_HIP(hipMemcpyAsync(h_ls.data(), ls, sizeof(unsigned long long) * 3 * kLanes,
+                            hipMemcpyDeviceToHost, s));
+    const int levels_now = ws->h_state->levels;
+    if (stats && levels_now > 0)
+      DAWN_HIP(hipMemcpyAsync(h_modes.data(), ws->trace_mode.p,
+                              sizeof(int32_t) * std::min(levels_now + 1, kTraceMax),
+                              hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    DAWN_HIP(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+    total_ms += ms;
+    if (dist && !dist_dev) {
+      DAWN_HIP(hipMemcpyAsync(dist + (int64_t)b0 * g->n, ws->distbuf.p,
+                              sizeof(int32_t) * (size_t)lanes * g->n, hipMemcpyDeviceToHost, s));
+      DAWN_HIP(hipStreamSynchronize(s));
+    }
+    for (int i = 0; i < lanes; i++) {
+      if (sums) sums[b0 + i] = (int64_t)h_ls[i];
+      if (reached) reached[b0 + i] = (int64_t)h_ls[kLanes + i];
+      if (edges) edges[b0 + i] = (int64_t)h_ls[2 * kLanes + i];
+    }
+    if (stats) {
+      stats->batches += 1;
+      for (int i = 0; i < lanes; i++) {
+        stats->vertices_reached += (int64_t)h_ls[kLanes + i];
+        stats->edges_reached += (int64_t)h_ls[2 * kLanes + i];
+        const int32_t sv = ws->h_batch->src[i];
+        if (g->h_row_ptr[sv] == g->h_row_ptr[sv + 1]) stats->skipped_sources += 1;
+      }
+      const int levels = ws->h_state->levels;
+      for (int st = 1; st <= levels && st < kTraceMax; st++) {
+        stats->steps += 1;
+        const int md = h_modes[st];
+        if (md >= 0 && md < DAWN_NUM_MODES) stats->mode_steps[md] += 1;
+      }
+    }
+  }
+  if (stats) stats->device_ms += total_ms;
+  return DAWN_OK;
+}
+
+// Per-step mode timings from identical state (scheduler training data).
+// Each step: decide (features), snapshot, then every mode is run `repeats`
+// times from the snapshot and timed with HIP events; the minimum is kept.
+int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
+                  float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
+                  int32_t* rows_out, int32_t policy) {
+  if (!g || S <= 0 || !sources || repeats < 1) return DAWN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  DAWN_TRY(ensure_ms(g));
+  MsWorkspace* ws = g->ms;
+  DAWN_TRY(ws->seen_bak.alloc(ws->npad));
+  hipStream_t s = g->stream;
+  MsArgs a = make_args(g);
+  SchedParams p = make_params(g, policy);
+  SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
+  const unsigned grid = ms_grid(g);
+  unsigned long long* ls = ws->lane_stats.p;
+  MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 0};
+  int32_t rows = 0;
+  for (int32_t b0 = 0; b0 < S && rows < max_rows; b0 += kLanes) {
+    const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
+    stage_batch(ws, sources + b0, lanes);
+    DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+    ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o);
+    ms_seed_kernel<<<1, 64, 0, s>>>(a);
+    for (int step = 1; rows < max_rows && step < kHist; step++) {
+      ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+      DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
+      DAWN_HIP(hipStreamSynchronize(s));
+      if (ws->h_state->done) break;
+      const SchedState snap = *ws->h_state;
+      if (chosen) chosen[rows] = snap.mode;
+      DAWN_HIP(hipMemcpyAsync(features + (size_t)rows * 8, ws->trace_feat.p + (size_t)snap.step * 8,
+                              sizeof(float) * 8, hipMemcpyDeviceToHost, s));
+      DAWN_HIP(hipMemcpyAsync(ws->seen_bak.p, ws->seen.p, sizeof(uint64_t) * ws->npad,
+                              hipMemcpyDeviceToDevice, s));
+      DAWN_HIP(hipMemcpyAsync(ws->ring_bak.p, ws->ring.p, sizeof(LevelCounters) * kRing,
+                              hipMemcpyDeviceToDevice, s));
+      for (int mode = 0; mode < DAWN_NUM_MODES; mode++) {
+        float best = 1e30f;
+        for (int r = 0; r < repeats; r++) {
+          SchedState st2 = snap;
+          st2.mode = mode;
+          *ws->h_state = st2;
+          DAWN_HIP(hipMemcpyAsync(ws->st.p, ws->h_state, sizeof(SchedState), hipMemcpyHostToDevice, s));
+          DAWN_HIP(hipMemcpyAsync(ws->seen.p, ws->seen_bak.p, sizeof(uint64_t) * ws->npad,
+                                  hipMemcpyDeviceToDevice, s));
+          DAWN_HIP(hipMemcpyAsync(ws->ring.p, ws->ring_bak.p, sizeof(LevelCounters) * kRing,
+                                  hipMemcpyDeviceToDevice, s));
+          DAWN_HIP(hipEventRecord(ws->e_a, s));
+          ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+          ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
+          DAWN_HIP(hipEventRecord(ws->e_b, s));
+          DAWN_HIP(hipEventSynchronize(ws->e_b));
+          float ms = 0.f;
+          DAWN_HIP(hipEventElapsedTime(&ms, ws->e_a, ws->e_b));
+          best = std::min(best, ms);
+        }
+        times_ms[(size_t)rows * DAWN_NUM_MODES + mode] = best;
+      }
+      rows++;
+    }
+    DAWN_HIP(hipStreamSynchronize(s));
+  }
+  *rows_out = rows;
+  return DAWN_OK;
+}
+
+}  // namespace dawn

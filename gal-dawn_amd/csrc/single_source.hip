@@ -1,0 +1,706 @@
+// Single-source frontier engine: unweighted DAWN (BOVM) and int32 min-plus
+// DAWN (SOVM) on one HIP device.
+//
+// Reference: DAWN::BFS_GPU (src/algorithm/gpu/bfs.cu:9-129) and
+// DAWN::SSSP_GPU (src/algorithm/gpu/sssp.cu:18-157); CPU semantics
+// DAWN::BFS_CPU (src/algorithm/cpu/bfs.cpp:23-185) and DAWN::SSSP_CPU
+// (src/algorithm/cpu/sssp.cpp:23-199).
+//
+// Layout in HBM (per graph, reused across calls):
+//   dist   int32[n]                      distances (INF = unreached)
+//   q      int32[3][n]                   frontier queue of level l in slot l%3
+//   bm     uint32[3][words]              frontier bitmap of level l in slot l%3
+//   ring   LevelCounters[4]              per-level counters, level l in l%4
+//   state  SchedState                    step / done / mode / rule state
+// Every step writes BOTH the queue and the bitmap of the new level, so the
+// three modes are interchangeable from step to step with no conversion:
+//   PUSH    queue in: workgroup-balanced edge expansion (LDS degree scan)
+//   PULL    bitmap in: unvisited vertices scan in-edges (CSC) with early exit
+//   BITMAP  bitmap in: dense scan of the frontier bitmap, top-down expansion
+// The step loop is decide(1 wave) + expand(grid), captured K steps at a
+// time into a hipGraph; the host reads one flag per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+
+#include "common.hpp"
+#include "sched.hpp"
+
+namespace dawn {
+
+struct SsArgs {
+  int32_t n;
+  int32_t source;
+  int64_t words;  // bitmap words per level slot
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* w;
+  const int32_t* in_row_ptr;
+  const int32_t* in_col;
+  const int32_t* in_w;
+  int32_t* dist;
+  int32_t* q;        // 3 * n
+  uint32_t* bm;      // 3 * words
+  LevelCounters* ring;
+  SchedState* st;
+};
+
+struct SsWorkspace {
+  int32_t n = 0;
+  int64_t words = 0;
+  DevBuf<int32_t> dist, q;
+  DevBuf<uint32_t> bm;
+  DevBuf<LevelCounters> ring;
+  DevBuf<SchedState> st;
+  DevBuf<float> trace_feat;
+  DevBuf<int32_t> trace_mode;
+  DevBuf<unsigned long long> reach;  // [2]
+  SchedState* h_state = nullptr;     // pinned
+  // Captured K-step chunks keyed by (kind, policy, K).
+  std::map<long long, hipGraphExec_t> graphs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+void ss_workspace_free(SsWorkspace* ws) {
+  if (!ws) return;
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (ws->h_state) (void)hipHostFree(ws->h_state);
+  if (ws->ev0) (void)hipEventDestroy(ws->ev0);
+  if (ws->ev1) (void)hipEventDestroy(ws->ev1);
+  delete ws;
+}
+
+namespace {
+
+__device__ __forceinline__ bool test_bit(const uint32_t* bm, int32_t v) {
+  return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Largest j in [0, kBlock) with off[j] <= e (off non-decreasing).
+__device__ __forceinline__ int owner_of(const int* off, int e) {
+  int lo = 0, hi = kBlock - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct Acc {
+  unsigned long long fv = 0, fe = 0, scout = 0;
+  __device__ void add(const int32_t* row_ptr, int32_t v) {
+    int d = row_ptr[v + 1] - row_ptr[v];
+    fv += 1;
+    fe += (unsigned)d;
+    scout += (unsigned)(d > 0 ? d : 1);
+  }
+  __device__ void flush(LevelCounters* c) {
+    unsigned long long a = wave_sum_u64(fv), b = wave_sum_u64(fe), s = wave_sum_u64(scout);
+    if (lane_id() == 0 && a) {
+      atomicAdd(&c->fv, a);
+      atomicAdd(&c->fe, b);
+      atomicAdd(&c->scout, s);
+      atomicAdd(&c->flanes, a);
+    }
+  }
+};
+
+// Records a newly discovered vertex of the level: queue slot (wave
+// aggregated) and counters.  Called by every lane of the wave.
+__device__ __forceinline__ void emit(bool disc, int32_t v, int32_t* q_out,
+                                     unsigned* qlen, Acc& acc,
+                                     const int32_t* row_ptr) {
+  int slot = wave_append(qlen, disc);
+  if (disc) {
+    q_out[slot] = v;
+    acc.add(row_ptr, v);
+  }
+}
+
+// ---------------------------------------------------------------- kernels ---
+
+__global__ void ss_fill_kernel(SsArgs a, int weighted) {
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t0; i < a.n; i += gs) a.dist[i] = kInf;
+  for (int64_t i = t0; i < 3 * a.words; i += gs) a.bm[i] = 0u;
+  if (t0 < kRing) a.ring[t0] = LevelCounters{};
+  (void)weighted;
+}
+
+// Level 0 = {source}.  GAPBS starts scout_count at out_degree(source)
+// (test/gabps/bfs.cc:140).
+__global__ void bfs_seed_kernel(SsArgs a, int32_t policy) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int32_t s = a.source;
+  const int d = a.row_ptr[s + 1] - a.row_ptr[s];
+  a.dist[s] = 0;
+  a.bm[s >> 5] |= 1u << (s & 31);
+  a.q[0] = s;
+  LevelCounters c{};
+  c.fv = 1;
+  c.fe = (unsigned)d;
+  c.scout = (unsigned)d;
+  c.flanes = 1;
+  c.qlen = 1;
+  a.ring[0] = c;
+  SchedState st{};
+  st.step = 0;
+  st.e2c = a.row_ptr[a.n];
+  st.scout = d;
+  st.force_mode = -1;
+  st.last_mode = DAWN_MODE_PUSH;
+  *a.st = st;
+  (void)policy;
+}
+
+// Level 1 for min-plus, as SSSP_CPU::SSSP initialises it
+// (src/algorithm/cpu/sssp.cpp:40-46): every listed out-edge of the source
+// writes its weight in row order (a later duplicate overwrites an earlier
+// one), the distinct neighbours form the first frontier, distance[source]
+// = 0, and the source is never relaxed afterwards.
+__global__ void sssp_seed_kernel(SsArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int32_t s = a.source;
+  const int32_t b = a.row_ptr[s], e = a.row_ptr[s + 1];
+  for (int32_t k = b; k < e; k++) a.dist[a.col[k]] = a.w[k];
+  a.dist[s] = 0;
+  uint32_t* bm1 = a.bm + a.words;  // level 1 slot
+  int32_t* q1 = a.q + a.n;
+  LevelCounters c{};
+  for (int32_t k = b; k < e; k++) {
+    int32_t v = a.col[k];
+    if (v == s) continue;
+    uint32_t bit = 1u << (v & 31);
+    if (bm1[v >> 5] & bit) continue;
+    bm1[v >> 5] |= bit;
+    q1[c.qlen++] = v;
+    int d = a.row_ptr[v + 1] - a.row_ptr[v];
+    c.fv += 1;
+    c.fe += (unsigned)d;
+    c.scout += (unsigned)(d > 0 ? d : 1);
+  }
+  c.flanes = c.fv;
+  a.ring[1] = c;
+  SchedState st{};
+  st.step = 1;
+  st.e2c = a.row_ptr[a.n];
+  st.scout = (long long)c.scout;
+  st.force_mode = -1;
+  st.last_mode = DAWN_MODE_PUSH;
+  *a.st = st;
+}
+
+__global__ void ss_decide_kernel(SsArgs a, SchedParams p, SchedTrace tr) {
+  sched_decide(a.st, a.ring, p, tr);
+}
+
+// One BOVM step: discovers level lv = st->step.
+__global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
+  __shared__ int s_off[kBlock];
+  __shared__ int s_beg[kBlock];
+  __shared__ int s_scan[kBlock / kWave];
+  const SchedState st = *a.st;
+  if (st.done) return;
+  const int lv = st.step;
+  const int pi = (lv + 2) % 3, ci = lv % 3, ni = (lv + 1) % 3;
+  const uint32_t* bm_in = a.bm + pi * a.words;
+  uint32_t* bm_out = a.bm + ci * a.words;
+  uint32_t* bm_nx = a.bm + ni * a.words;
+  const int32_t* q_in = a.q + (int64_t)pi * a.n;
+  int32_t* q_out = a.q + (int64_t)ci * a.n;
+  LevelCounters* c_out = a.ring + (lv % kRing);
+  unsigned* qlen = &c_out->qlen;
+  const int lane = lane_id();
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = gt; i < a.words; i += gs) bm_nx[i] = 0u;
+  if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
+  Acc acc;
+
+  if (st.mode == DAWN_MODE_PUSH) {
+    const unsigned qn = a.ring[(lv + kRing - 1) % kRing].qlen;
+    for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
+      const int64_t i = c0 + threadIdx.x;
+      int beg = 0, deg = 0;
+      if (i < qn) {
+        int u = q_in[i];
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+      }
+      int total;
+      int off = block_exclusive_scan(deg, s_scan, &total);
+      s_off[threadIdx.x] = off;
+      s_beg[threadIdx.x] = beg;
+      __syncthreads();
+      for (int e0 = 0; e0 < total; e0 += kBlock) {
+        const int e = e0 + threadIdx.x;
+        bool disc = false;
+        int32_t v = 0;
+        if (e < total) {
+          int j = owner_of(s_off, e);
+          v = a.col[s_beg[j] + (e - s_off[j])];
+          if (a.dist[v] == kInf && atomicCAS(&a.dist[v], kInf, lv) == kInf) {
+            disc = true;
+            atomicOr(&bm_out[v >> 5], 1u << (v & 31));
+          }
+        }
+        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+      }
+      __syncthreads();
+    }
+  } else if (st.mode == DAWN_MODE_PULL) {
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+         base += wstride) {
+      const int32_t v = (int32_t)(base + lane);
+      bool cand = v < a.n && a.dist[v] == kInf;
+      int beg = 0, deg = 0;
+      if (cand) {
+        beg = a.in_row_ptr[v];
+        deg = a.in_row_ptr[v + 1] - beg;
+      }
+      const bool light = cand && deg < kHeavyDegree;
+      const int maxd = wave_max_i32(light ? deg : 0);
+      bool found = false;
+      for (int j = 0; j < maxd; j++) {
+        if (light && !found && j < deg) found = test_bit(bm_in, a.in_col[beg + j]);
+      }
+      uint64_t hm = __ballot(cand && deg >= kHeavyDegree);
+      while (hm) {
+        const int l = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
+        bool f = false;
+        for (int k0 = hb; k0 < he; k0 += kWave) {
+          const int k = k0 + lane;
+          const bool b = k < he && test_bit(bm_in, a.in_col[k]);
+          if (__ballot(b)) { f = true; break; }
+        }
+        if (lane == l) found = f;
+      }
+      if (found) a.dist[v] = lv;
+      const uint64_t fm = __ballot(found);
+      if (lane == 0) {
+        bm_out[base >> 5] = (uint32_t)fm;
+        bm_out[(base >> 5) + 1] = (uint32_t)(fm >> 32);
+      }
+      emit(found, v, q_out, qlen, acc, a.row_ptr);
+    }
+  } else {  // DAWN_MODE_BITMAP: dense top-down over the frontier bitmap
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+         base += wstride) {
+      const int32_t u = (int32_t)(base + lane);
+      const bool act = u < a.n && test_bit(bm_in, u);
+      if (__ballot(act) == 0) continue;
+      int beg = 0, deg = 0;
+      if (act) {
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+      }
+      const bool light = act && deg < kHeavyDegree;
+      const int maxd = wave_max_i32(light ? deg : 0);
+      for (int j = 0; j < maxd; j++) {
+        bool disc = false;
+        int32_t v = 0;
+        if (light && j < deg) {
+          v = a.col[beg + j];
+          if (a.dist[v] == kInf && atomicCAS(&a.dist[v], kInf, lv) == kInf) {
+            disc = true;
+            atomicOr(&bm_out[v >> 5], 1u << (v & 31));
+          }
+        }
+        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+      }
+      uint64_t hm = __ballot(act && deg >= kHeavyDegree);
+      while (hm) {
+        const int l = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
+        for (int k0 = hb; k0 < he; k0 += kWave) {
+          const int k = k0 + lane;
+          bool disc = false;
+          int32_t v = 0;
+          if (k < he) {
+            v = a.col[k];
+            if (a.dist[v] == kInf && atomicCAS(&a.dist[v], kInf, lv) == kInf) {
+              disc = true;
+              atomicOr(&bm_out[v >> 5], 1u << (v & 31));
+            }
+          }
+          emit(disc, v, q_out, qlen, acc, a.row_ptr);
+        }
+      }
+    }
+  }
+  acc.flush(c_out);
+}
+
+// Relaxation of edge (u -> v, w) with du = current distance of u.  Returns
+// true when v joins the next frontier for the first time this level.
+__device__ __forceinline__ bool relax(const SsArgs& a, int32_t du, int32_t v,
+                                      int32_t w, uint32_t* bm_out) {
+  if (v == a.source) return false;
+  const long long nd = (long long)du + (long long)w;
+  if (nd >= kInf) return false;
+  const int32_t ndi = (int32_t)nd;
+  if (ndi >= a.dist[v]) return false;
+  const int32_t old = atomicMin(&a.dist[v], ndi);
+  if (ndi >= old) return false;
+  const uint32_t bit = 1u << (v & 31);
+  const uint32_t prev = atomicOr(&bm_out[v >> 5], bit);
+  return (prev & bit) == 0;
+}
+
+// One min-plus step (GOVM/GOVMP semantics): relaxes the out-edges of the
+// vertices whose distance dropped in the previous step.
+__global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
+  __shared__ int s_off[kBlock];
+  __shared__ int s_beg[kBlock];
+  __shared__ int s_du[kBlock];
+  __shared__ int s_scan[kBlock / kWave];
+  const SchedState st = *a.st;
+  if (st.done) return;
+  const int lv = st.step;
+  const int pi = (lv + 2) % 3, ci = lv % 3, ni = (lv + 1) % 3;
+  const uint32_t* bm_in = a.bm + pi * a.words;
+  uint32_t* bm_out = a.bm + ci * a.words;
+  uint32_t* bm_nx = a.bm + ni * a.words;
+  const int32_t* q_in = a.q + (int64_t)pi * a.n;
+  int32_t* q_out = a.q + (int64_t)ci * a.n;
+  LevelCounters* c_out = a.ring + (lv % kRing);
+  unsigned* qlen = &c_out->qlen;
+  const int lane = lane_id();
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = gt; i < a.words; i += gs) bm_nx[i] = 0u;
+  if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
+  Acc acc;
+
+  if (st.mode == DAWN_MODE_PUSH) {
+    const unsigned qn = a.ring[(lv + kRing - 1) % kRing].qlen;
+    for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
+      const int64_t i = c0 + threadIdx.x;
+      int beg = 0, deg = 0, du = 0;
+      if (i < qn) {
+        int u = q_in[i];
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+        du = a.dist[u];
+      }
+      int total;
+      int off = block_exclusive_scan(deg, s_scan, &total);
+      s_off[threadIdx.x] = off;
+      s_beg[threadIdx.x] = beg;
+      s_du[threadIdx.x] = du;
+      __syncthreads();
+      for (int e0 = 0; e0 < total; e0 += kBlock) {
+        const int e = e0 + threadIdx.x;
+        bool disc = false;
+        int32_t v = 0;
+        if (e < total) {
+          int j = owner_of(s_off, e);
+          int k = s_beg[j] + (e - s_off[j]);
+          v = a.col[k];
+          disc = relax(a, s_du[j], v, a.w[k], bm_out);
+        }
+        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+      }
+      __syncthreads();
+    }
+  } else if (st.mode == DAWN_MODE_PULL) {
+    // Every vertex takes the minimum over its in-edges from frontier
+    // vertices; each vertex is written only by its own lane.
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+         base += wstride) {
+      const int32_t v = (int32_t)(base + lane);
+      const bool cand = v < a.n && v != a.source;
+      int beg = 0, deg = 0;
+      int32_t cur = kInf;
+      if (cand) {
+        beg = a.in_row_ptr[v];
+        deg = a.in_row_ptr[v + 1] - beg;
+        cur = a.dist[v];
+      }
+      int32_t best = cur;
+      const bool light = cand && deg < kHeavyDegree;
+      const int maxd = wave_max_i32(light ? deg : 0);
+      for (int j = 0; j < maxd; j++) {
+        if (light && j < deg) {
+          const int32_t u = a.in_col[beg + j];
+          if (test_bit(bm_in, u)) {
+            long long nd = (long long)a.dist[u] + a.in_w[beg + j];
+            if (nd < best) best = (int32_t)nd;
+          }
+        }
+      }
+      uint64_t hm = __ballot(cand && deg >= kHeavyDegree);
+      while (hm) {
+        const int l = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
+        int32_t hbest = kInf;
+        for (int k = hb + lane; k < he; k += kWave) {
+          const int32_t u = a.in_col[k];
+          if (test_bit(bm_in, u)) {
+            long long nd = (long long)a.dist[u] + a.in_w[k];
+            if (nd < hbest) hbest = (int32_t)nd;
+          }
+        }
+        for (int o = 32; o >= 1; o >>= 1) hbest = min(hbest, __shfl_xor(hbest, o));
+        if (lane == l && hbest < best) best = hbest;
+      }
+      const bool found = cand && best < cur;
+      if (found) a.dist[v] = best;
+      const uint64_t fm = __ballot(found);
+      if (lane == 0) {
+        bm_out[base >> 5] = (uint32_t)fm;
+        bm_out[(base >> 5) + 1] = (uint32_t)(fm >> 32);
+      }
+      emit(found, v, q_out, qlen, acc, a.row_ptr);
+    }
+  } else {  // DAWN_MODE_BITMAP (GOVMP shape: dense scan of the frontier)
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+         base += wstride) {
+      const int32_t u = (int32_t)(base + lane);
+      const bool act = u < a.n && test_bit(bm_in, u);
+      if (__ballot(act) == 0) continue;
+      int beg = 0, deg = 0;
+      int32_t du = 0;
+      if (act) {
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+        du = a.dist[u];
+      }
+      const bool light = act && deg < kHeavyDegree;
+      const int maxd = wave_max_i32(light ? deg : 0);
+      for (int j = 0; j < maxd; j++) {
+        bool disc = false;
+        int32_t v = 0;
+        if (light && j < deg) {
+          v = a.col[beg + j];
+          disc = relax(a, du, v, a.w[beg + j], bm_out);
+        }
+        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+      }
+      uint64_t hm = __ballot(act && deg >= kHeavyDegree);
+      while (hm) {
+        const int l = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
+        const int32_t hdu = __shfl(du, l);
+        for (int k0 = hb; k0 < he; k0 += kWave) {
+          const int k = k0 + lane;
+          bool disc = false;
+          int32_t v = 0;
+          if (k < he) {
+            v = a.col[k];
+            disc = relax(a, hdu, v, a.w[k], bm_out);
+          }
+          emit(disc, v, q_out, qlen, acc, a.row_ptr);
+        }
+      }
+    }
+  }
+  acc.flush(c_out);
+}
+
+// Sum of out-degree and count of reached vertices (GTEPS numerator).
+__global__ void reach_kernel(const int32_t* dist, const int32_t* row_ptr, int32_t n,
+                             unsigned long long* out) {
+  unsigned long long cnt = 0, edges = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (dist[v] != kInf) {
+      cnt++;
+      edges += (unsigned)(row_ptr[v + 1] - row_ptr[v]);
+    }
+  }
+  cnt = wave_sum_u64(cnt);
+  edges = wave_sum_u64(edges);
+  if (lane_id() == 0 && cnt) {
+    atomicAdd(&out[0], cnt);
+    atomicAdd(&out[1], edges);
+  }
+}
+
+int ensure_ss(dawn_graph* g) {
+  if (!g->ss) g->ss = new SsWorkspace();
+  SsWorkspace* ws = g->ss;
+  if (ws->n == g->n && ws->dist.p) return DAWN_OK;
+  ws->n = g->n;
+  ws->words = ((int64_t)g->n + 63) / 64 * 2;
+  DAWN_TRY(ws->dist.alloc(g->n));
+  DAWN_TRY(ws->q.alloc((size_t)3 * g->n));
+  DAWN_TRY(ws->bm.alloc((size_t)3 * ws->words));
+  DAWN_TRY(ws->ring.alloc(kRing));
+  DAWN_TRY(ws->st.alloc(1));
+  DAWN_TRY(ws->trace_feat.alloc((size_t)kTraceMax * 8));
+  DAWN_TRY(ws->trace_mode.alloc(kTraceMax));
+  DAWN_TRY(ws->reach.alloc(2));
+  if (!ws->h_state) DAWN_HIP(hipHostMalloc((void**)&ws->h_state, sizeof(SchedState)));
+  if (!ws->ev0) DAWN_HIP(hipEventCreate(&ws->ev0));
+  if (!ws->ev1) DAWN_HIP(hipEventCreate(&ws->ev1));
+  return DAWN_OK;
+}
+
+}  // namespace
+
+// Runs one single-source traversal.  kind 0: BOVM, 1: min-plus.
+// steps_per_chunk: decide+expand pairs per captured graph.
+int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out,
+                      int32_t policy, dawn_stats* stats, int32_t force_mode,
+                      float* trace_feat, int32_t* trace_mode, int32_t trace_max,
+                      int32_t* trace_rows) {
+  if (!g || source < 0 || source >= g->n) return DAWN_ERR_INVALID_ARG;
+  if (kind == 1 && !g->weighted) return DAWN_ERR_NOT_WEIGHTED;
+  if (g->h_row_ptr[source] == g->h_row_ptr[source + 1]) return DAWN_ERR_ISOLATED_SOURCE;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  DAWN_TRY(ensure_ss(g));
+  SsWorkspace* ws = g->ss;
+  hipStream_t s = g->stream;
+
+  SsArgs a{};
+  a.n = g->n;
+  a.source = source;
+  a.words = ws->words;
+  a.row_ptr = g->row_ptr;
+  a.col = g->col;
+  a.w = g->weight;
+  a.in_row_ptr = g->in_row_ptr;
+  a.in_col = g->in_col;
+  a.in_w = g->in_weight;
+  a.dist = ws->dist.p;
+  a.q = ws->q.p;
+  a.bm = ws->bm.p;
+  a.ring = ws->ring.p;
+  a.st = ws->st.p;
+
+  SchedParams p{};
+  p.n = g->n;
+  p.m = g->m;
+  p.policy = policy;
+  p.alpha = 15;
+  p.beta = 18;
+  p.batched = 0;
+  p.allow_pull = 1;
+  p.avg_degree = g->n ? (float)((double)g->m / g->n) : 0.f;
+  p.max_degree = (float)g->max_out_degree;
+  p.mlp = sched_mlp_device(g->device);
+  p.mlp_norm = sched_norm_device(g->device);
+  if (kind == 1 && policy == DAWN_POLICY_LEARNED) p.policy = DAWN_POLICY_FIXED;
+  SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
+
+  const unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock, 4096);
+  // A chunk = K decide+expand pairs, captured once per (kind, policy, mode, K).
+  const int K = g->n < (1 << 20) ? 8 : 16;
+  const long long key = ((long long)kind << 40) | ((long long)(policy & 0xff) << 32) |
+                        ((long long)(force_mode + 1) << 16) | K;
+  hipGraphExec_t exec = nullptr;
+  auto it = ws->graphs.find(key);
+  if (it != ws->graphs.end()) {
+    exec = it->second;
+  } else {
+    hipGraph_t graph;
+    DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < K; i++) {
+      ss_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+      if (kind == 0)
+        bfs_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+      else
+        sssp_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+    }
+    DAWN_HIP(hipStreamEndCapture(s, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DAWN_HIP(e);
+    ws->graphs[key] = exec;
+  }
+
+  DAWN_HIP(hipEventRecord(ws->ev0, s));
+  ss_fill_kernel<<<grid_for(std::max<int64_t>(3 * ws->words, g->n)), kBlock, 0, s>>>(a, kind);
+  if (kind == 0)
+    bfs_seed_kernel<<<1, 64, 0, s>>>(a, policy);
+  else
+    sssp_seed_kernel<<<1, 64, 0, s>>>(a);
+  DAWN_HIP(hipGetLastError());
+  if (force_mode >= 0) {
+    // The seed kernel leaves force_mode = -1; set it before stepping.
+    const int32_t fm = force_mode;
+    DAWN_HIP(hipMemcpyAsync(&ws->st.p->force_mode, &fm, sizeof(int32_t),
+                            hipMemcpyHostToDevice, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+  }
+  int64_t launched = 0;
+  const int64_t limit = (int64_t)g->n + 2;
+  for (;;) {
+    DAWN_HIP(hipGraphLaunch(exec, s));
+    launched += K;
+    DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState),
+                            hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    if (ws->h_state->done || launched > limit) break;
+  }
+  DAWN_HIP(hipEventRecord(ws->ev1, s));
+  DAWN_HIP(hipEventSynchronize(ws->ev1));
+  float ms = 0.f;
+  DAWN_HIP(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+  if (!ws->h_state->done) return DAWN_ERR_HIP;  // step bound exceeded
+
+  const int levels = ws->h_state->levels;
+  const int first_step = kind == 0 ? 1 : 2;
+  if (stats) {
+    DAWN_HIP(hipMemsetAsync(ws->reach.p, 0, 2 * sizeof(unsigned long long), s));
+    reach_kernel<<<grid_for(g->n), kBlock, 0, s>>>(ws->dist.p, g->row_ptr, g->n, ws->reach.p);
+    unsigned long long rr[2];
+    DAWN_HIP(hipMemcpyAsync(rr, ws->reach.p, sizeof rr, hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> modes(std::max(levels, 1));
+    int nrec = std::min(levels, kTraceMax);
+    if (nrec > 0)
+      DAWN_HIP(hipMemcpyAsync(modes.data(), ws->trace_mode.p, sizeof(int32_t) * nrec,
+                              hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    stats->device_ms += ms;
+    stats->vertices_reached += (int64_t)rr[0];
+    stats->edges_reached += (int64_t)rr[1];
+    stats->batches += 1;
+    // Steps first_step..levels expanded a level; the final step found none.
+    for (int st = first_step; st <= levels && st < kTraceMax; st++) {
+      stats->steps += 1;
+      int md = modes[st];
+      if (md >= 0 && md < DAWN_NUM_MODES) stats->mode_steps[md] += 1;
+    }
+  }
+  if (trace_rows) {
+    int rows = 0;
+    int nrec = std::min(levels + 1, kTraceMax);
+    std::vector<int32_t> modes(nrec);
+    std::vector<float> feat((size_t)nrec * 8);
+    DAWN_HIP(hipMemcpyAsync(modes.data(), ws->trace_mode.p, sizeof(int32_t) * nrec,
+                            hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipMemcpyAsync(feat.data(), ws->trace_feat.p, sizeof(float) * 8 * nrec,
+                            hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    for (int st = first_step; st <= levels && st < nrec && rows < trace_max; st++, rows++) {
+      if (trace_mode) trace_mode[rows] = modes[st];
+      if (trace_feat)
+        for (int i = 0; i < 8; i++) trace_feat[rows * 8 + i] = feat[(size_t)st * 8 + i];
+    }
+    *trace_rows = rows;
+  }
+  if (dist_out) {
+    const bool dev = is_device_pointer(dist_out);
+    DAWN_HIP(hipMemcpyAsync(dist_out, ws->dist.p, sizeof(int32_t) * g->n,
+                            dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+  }
+  return DAWN_OK;
+}
+
+}  // namespace dawn

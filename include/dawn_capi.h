@@ -1,0 +1,245 @@
+/*
+ * dawn_capi.h — C ABI of the MI355X-native DAWN frontier engine.
+ *
+ * Plain C types only (pointers + sizes); no HIP or torch types cross this
+ * boundary.  Every compute entry point returns a status code (DAWN_OK == 0)
+ * instead of calling exit(); the C++ drop-in layer (dawn_amd.hxx) maps the
+ * isolated-source status back to the reference's behaviour.
+ *
+ * Distance conventions (all entry points):
+ *   dist[source] = 0, dist[v] = hop count (BFS) or int32 path weight (SSSP),
+ *   unreachable vertices = DAWN_INF (INT32_MAX).
+ * The reference writes 0 for unreachable vertices in its unweighted path
+ * (src/algorithm/cpu/bfs.cpp:38,62) and FLT_MAX in its weighted path
+ * (src/algorithm/cpu/sssp.cpp:40); dawn_write_dist() reproduces its output
+ * file format from these arrays.
+ */
+#ifndef DAWN_CAPI_H
+#define DAWN_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DAWN_INF 2147483647
+
+/* Status codes. */
+enum {
+  DAWN_OK = 0,
+  DAWN_ERR_INVALID_ARG = -1,
+  DAWN_ERR_ISOLATED_SOURCE = -2, /* source has no out-edges (reference: exit(0)) */
+  DAWN_ERR_NO_DEVICE = -3,
+  DAWN_ERR_HIP = -4,
+  DAWN_ERR_OUT_OF_MEMORY = -5,
+  DAWN_ERR_IO = -6,          /* file cannot be opened (reference: cerr + return) */
+  DAWN_ERR_FORMAT = -7,      /* not a MatrixMarket coordinate file / bad weight */
+  DAWN_ERR_OVERFLOW = -8,    /* sizes beyond int32 CSR offsets */
+  DAWN_ERR_UNSUPPORTED = -9,
+  DAWN_ERR_NOT_WEIGHTED = -10
+};
+
+/* Step modes of the frontier engine.  Every mode produces identical
+ * distances; the choice changes only time. */
+enum {
+  DAWN_MODE_PUSH = 0,   /* sparse frontier queue, top-down scatter          */
+  DAWN_MODE_PULL = 1,   /* bottom-up gather over unvisited vertices (CSC)   */
+  DAWN_MODE_BITMAP = 2  /* dense top-down scan of the frontier bitmap/words */
+};
+#define DAWN_NUM_MODES 3
+
+/* Scheduling policies. */
+enum {
+  DAWN_POLICY_FIXED = 0,   /* GAPBS alpha/beta rule (test/gabps/bfs.cc:146,161) */
+  DAWN_POLICY_LEARNED = 1, /* FP32 MLP 8->32->32->3 (falls back to FIXED if unset) */
+  DAWN_POLICY_PUSH = 2,    /* force one mode for every step */
+  DAWN_POLICY_PULL = 3,
+  DAWN_POLICY_BITMAP = 4
+};
+
+/* Host-side CSR, owned by the library when returned by a generator or
+ * reader (free with dawn_csr_free), borrowed when passed in. */
+typedef struct dawn_csr {
+  int32_t n;          /* vertices (reference Graph_t::rows)               */
+  int64_t m;          /* directed edges (reference Graph_t::nnz)          */
+  int32_t directed;   /* 0: symmetric adjacency                           */
+  int32_t weighted;   /* weight != NULL                                   */
+  int32_t* row_ptr;   /* [n+1]                                            */
+  int32_t* col;       /* [m]                                              */
+  int32_t* weight;    /* [m] or NULL                                      */
+} dawn_csr;
+
+/* Per-call statistics. */
+typedef struct dawn_stats {
+  double device_ms;          /* HIP-event time of the traversal            */
+  int64_t steps;             /* frontier steps executed (summed over batches) */
+  int64_t mode_steps[DAWN_NUM_MODES];
+  int64_t edges_reached;     /* sum over sources of out-degree of reached vertices */
+  int64_t vertices_reached;  /* sum over sources of reached vertices (incl. source) */
+  int64_t batches;
+  int32_t skipped_sources;   /* isolated sources (reference skips them, mssp.cpp:16-18) */
+  int32_t reserved;
+} dawn_stats;
+
+/* ---------------------------------------------------------------- host --- */
+
+/* Library / device info.  Returns DAWN_OK and the number of visible GPUs. */
+int dawn_device_count(int32_t* count);
+const char* dawn_status_string(int status);
+/* Text of the last HIP failure seen by this thread ("" if none). */
+const char* dawn_last_error(void);
+
+/* Seeded synthetic generators (deterministic for any thread count).
+ * All produce symmetrized, deduplicated CSR without self-loops; when
+ * weighted, weights are uniform integers in [wmin, wmax], equal in both
+ * directions, duplicates keep the minimum. */
+int dawn_gen_er(int32_t n, double avg_degree, uint64_t seed, int32_t weighted,
+                int32_t wmin, int32_t wmax, dawn_csr* out);
+int dawn_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                  double c, uint64_t seed, int32_t weighted, int32_t wmin,
+                  int32_t wmax, dawn_csr* out);
+int dawn_gen_grid(int32_t side, double extra_fraction, uint64_t seed,
+                  int32_t weighted, int32_t wmin, int32_t wmax, dawn_csr* out);
+/* Uniform sources in [0, n) from a seeded stream. */
+int dawn_gen_sources(int32_t n, int32_t count, uint64_t seed, int32_t* out);
+
+/* Build a CSR from an edge list (u[i] -> v[i], optional w[i]).
+ * symmetrize: add v->u; dedup: drop repeated (u,v) keeping min weight.
+ * Self-loops are always dropped (reference src/io.cpp:19,71,111,172). */
+int dawn_csr_from_edges(int32_t n, int64_t count, const int32_t* u,
+                        const int32_t* v, const int32_t* w, int32_t symmetrize,
+                        int32_t dedup, dawn_csr* out);
+
+/* MatrixMarket reader with the reference's semantics: pattern => unweighted,
+ * symmetric => undirected (both directions pushed), 1-based -> 0-based,
+ * self-loops dropped, duplicates KEPT, rows sorted by (col, weight).
+ * Reference: DAWN::Graph::createGraph (src/graph.cpp:9-77) and
+ * DAWN::IO::readGraph* (src/io.cpp:2-192).  Weights must be integral. */
+int dawn_read_mtx(const char* path, int32_t weighted, dawn_csr* out);
+/* Write a CSR as a MatrixMarket coordinate file (general/symmetric). */
+int dawn_write_mtx(const char* path, const dawn_csr* csr);
+void dawn_csr_free(dawn_csr* csr);
+
+/* Source list: one int per line, '%' lines skipped (src/io.cpp:194-213).
+ * *out is malloc'd; free with dawn_free. */
+int dawn_read_list(const char* path, int32_t** out, int64_t* count);
+void dawn_free(void* p);
+/* "src dst dist" lines for reached non-source vertices, the format of
+ * DAWN::IO::outfile (src/io.cpp:215-231). */
+int dawn_write_dist(const char* path, int32_t n, const int32_t* dist,
+                    int32_t source);
+
+/* -------------------------------------------------------------- device --- */
+
+typedef struct dawn_graph dawn_graph;
+
+/* Upload a CSR to the current HIP device; builds the in-edge CSC for
+ * directed graphs and the degree bins used by the step kernels. */
+int dawn_graph_create(const dawn_csr* csr, dawn_graph** out);
+int dawn_graph_destroy(dawn_graph* g);
+int dawn_graph_info(const dawn_graph* g, int32_t* n, int64_t* m,
+                    int32_t* directed, int32_t* weighted);
+
+/* Single-source unweighted DAWN (BOVM).  dist: n int32, host or device
+ * pointer.  Reference: DAWN::BFS_GPU::run (src/algorithm/gpu/bfs.cu:107-129),
+ * DAWN::BFS_CPU::run (src/algorithm/cpu/bfs.cpp:9-20). */
+int dawn_bfs(const dawn_graph* g, int32_t source, int32_t* dist,
+             int32_t policy, dawn_stats* stats);
+
+/* Single-source int32 min-plus DAWN (SOVM).  Reference: DAWN::SSSP_GPU::run
+ * (src/algorithm/gpu/sssp.cu:131-157), DAWN::SSSP_CPU::run
+ * (src/algorithm/cpu/sssp.cpp:9-21). */
+int dawn_sssp(const dawn_graph* g, int32_t source, int32_t* dist,
+              int32_t policy, dawn_stats* stats);
+
+/* Multi-source shortest paths.  Unweighted graphs run 64 sources per
+ * bit-packed batch; weighted graphs run the min-plus engine per source.
+ * sources are taken modulo n (reference: msource[i] % rows, mssp.cpp:15);
+ * isolated sources are skipped (dist row left INF except dist[src]=0, sum 0).
+ * dist: S*n int32 row-major (host or device), or NULL to keep only sums.
+ * sums: S int64 sums of distances over reached vertices, or NULL.
+ * Reference: DAWN::MSSP_GPU::run / run_Weighted (src/algorithm/gpu/mssp.cu:9-102),
+ * DAWN::MSSP_CPU::run (src/algorithm/cpu/mssp.cpp:9-32). */
+int dawn_mssp(const dawn_graph* g, const int32_t* sources, int32_t count,
+              int32_t* dist, int64_t* sums, int32_t policy, dawn_stats* stats);
+
+/* All-pairs: sources [first, first+count) streamed in batches; dist is
+ * count*n (host/device) or NULL.  Reference: DAWN::APSP_GPU::run
+ * (src/algorithm/gpu/apsp.cu:62-111). */
+int dawn_apsp_block(const dawn_graph* g, int32_t first, int32_t count,
+                    int32_t* dist, int64_t* sums, int32_t policy,
+                    dawn_stats* stats);
+
+/* Closeness centrality (n-1)/sum(dist), sum exact in int64, one IEEE double
+ * division.  closeness/sums: count entries (host).  ref_int: if non-NULL,
+ * receives the reference's unweighted value (row-1)/int32-sum with integer
+ * division (src/algorithm/cpu/cc.cpp:77-78) as float.
+ * Reference: DAWN::CC_GPU::run (src/algorithm/gpu/cc.cu:9-103). */
+int dawn_closeness(const dawn_graph* g, const int32_t* sources, int32_t count,
+                   double* closeness, int64_t* sums, float* ref_int,
+                   int32_t policy, dawn_stats* stats);
+
+/* Uneven batch: many independent graphs in one persistent launch.
+ * graphs[i] are device graphs; sources are count_per_graph per graph,
+ * concatenated.  dist_offsets[i] = offset of graph i's first row in dist
+ * (rows of n_i); dist may be host or device.  Weighted graphs run min-plus. */
+int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
+                    const int32_t* sources, int32_t count_per_graph,
+                    int32_t* dist, const int64_t* dist_offsets,
+                    dawn_stats* stats);
+
+/* ----------------------------------------------------------- scheduler --- */
+
+/* Number of per-step features read by the scheduler MLP. */
+#define DAWN_SCHED_FEATURES 8
+#define DAWN_SCHED_HIDDEN 32
+/* Parameter count of the 8->32->32->3 MLP: 8*32+32 + 32*32+32 + 32*3+3. */
+#define DAWN_SCHED_PARAMS 1443
+
+/* Install / clear the process-wide scheduler network (1443 floats, layout:
+ * W1[32][8], b1[32], W2[32][32], b2[32], W3[3][32], b3[3]) and the feature
+ * normalisation (mean[8], inv_std[8]).  params == NULL clears it. */
+int dawn_sched_set_mlp(const float* params, const float* feat_mean,
+                       const float* feat_inv_std);
+
+/* Record per-step mode timings: for each batch step every mode is run from
+ * the same state and timed with HIP events; then the engine advances.
+ * features: rows*8 floats, times_ms: rows*3 floats.  *rows receives the
+ * number of rows written (<= max_rows).  Unweighted graphs only. */
+int dawn_sched_record(const dawn_graph* g, const int32_t* sources,
+                      int32_t count, int32_t repeats, float* features,
+                      float* times_ms, int32_t max_rows, int32_t* rows);
+
+/* Per-step trace of a policy run: modes[i] and features (rows*8). */
+int dawn_sched_trace(const dawn_graph* g, const int32_t* sources,
+                     int32_t count, int32_t policy, int32_t* modes,
+                     float* features, float* times_ms, int32_t max_rows,
+                     int32_t* rows);
+
+/* MLP on the GPU (hand-written HIP kernels).  x: N*8, y: N*3 targets
+ * (log step time per mode; the network regresses them), params: 1443.
+ * dawn_mlp_forward: out N*3.  dawn_mlp_loss_grad: mean-squared loss and its
+ * gradient w.r.t. params (1443).  dawn_mlp_train: Adam, in-place params. */
+int dawn_mlp_forward(const float* params, const float* x, int32_t N,
+                     float* out);
+int dawn_mlp_loss_grad(const float* params, const float* x, const float* y,
+                       int32_t N, float* loss, float* grad);
+int dawn_mlp_train(float* params, const float* x, const float* y, int32_t N,
+                   int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                   float* final_loss);
+
+/* ------------------------------------------------------------ autotune --- */
+
+/* Autotuned launch knobs per (kernel, graph class).  dawn_tune runs a sweep
+ * on g and stores the winners in the in-process cache; load/save persist
+ * the cache as text. */
+int dawn_tune(const dawn_graph* g, int32_t verbose);
+int dawn_tune_load(const char* path);
+int dawn_tune_save(const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAWN_CAPI_H */

@@ -1,0 +1,184 @@
+"""HIP engine vs the CPU oracle, bit-exact, through the C ABI.
+
+Sizes: the small cases run every mode/policy; the BASELINE configurations
+(ER n=65,536 seed 1; 2048^2 grid + 1% long-range seed 2; RMAT scale 20
+seed 3) run at full size with the oracle on sampled sources plus
+size-independent checks (BFS level equations, symmetry of undirected
+distances, sums == row sums).
+"""
+import numpy as np
+import pytest
+
+from conftest import dawn
+
+INF = dawn.INF
+pytestmark = pytest.mark.gpu
+POLICIES = ["fixed", "push", "pull", "bitmap", "learned"]
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def bfs_level_equations_hold(h, source, d):
+    """d is the BFS distance from source iff d[s]=0 and, for v != s,
+    d[v] = 1 + min over in-neighbours (INF if none is reached)."""
+    n = h.n
+    rp, col = h.row_ptr, h.col
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    # For undirected graphs in-neighbours == out-neighbours.
+    du = d[rows].astype(np.int64)
+    best = np.full(n, np.iinfo(np.int64).max)
+    np.minimum.at(best, col, du)
+    exp = np.where(best >= INF, INF, best + 1)
+    exp[source] = 0
+    return np.array_equal(exp, d.astype(np.int64))
+
+
+@pytest.fixture(scope="module")
+def er_c1():
+    log("case: ER n=65536 avg deg 8 seed 1")
+    h = dawn.HostGraph.er(65536, 8.0, 1)
+    return h, dawn.Graph(h)
+
+
+def test_bfs_c1_all_policies(er_c1, orc):
+    h, g = er_c1
+    srcs = [0] + list(dawn.gen_sources(h.n, 64, 1))
+    for i, s in enumerate(srcs):
+        if h.degrees()[s] == 0:
+            continue
+        ref = orc.bfs(h.row_ptr, h.col, s)
+        pols = POLICIES if i < 4 else ["fixed"]
+        for p in pols:
+            d = g.bfs(s, policy=p)
+            assert np.array_equal(d, ref), (s, p)
+
+
+def test_bfs_fixed_schedule_matches_gapbs(er_c1, orc):
+    h, g = er_c1
+    for s in [0, 17, 4242]:
+        modes = orc.dobfs_modes(h.row_ptr, h.col, s)
+        d, st = g.bfs(s, policy="fixed", stats=True)
+        assert st["steps"] == len(modes)
+        assert st["mode_steps"]["pull"] == int((modes == 1).sum())
+        assert st["mode_steps"]["push"] == int((modes == 0).sum())
+
+
+def test_mssp_c1_batched_equals_oracle(er_c1, orc):
+    h, g = er_c1
+    srcs = np.concatenate([[0], dawn.gen_sources(h.n, 64, 1)]).astype(np.int32)
+    for p in POLICIES:
+        dist, sums = g.mssp(srcs, policy=p)
+        odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+
+
+def test_sssp_small_grid_all_policies(orc):
+    log("case: grid 128x128 + 1% long-range, w in [1,1000], seed 2")
+    h = dawn.HostGraph.grid(128, 0.01, 2, weighted=True)
+    g = dawn.Graph(h)
+    for s in dawn.gen_sources(h.n, 4, 2):
+        ref = orc.sssp(h.row_ptr, h.col, h.weight, s, dense=False)
+        assert np.array_equal(ref, orc.dijkstra(h.row_ptr, h.col, h.weight, s))
+        for p in ["fixed", "push", "pull", "bitmap"]:
+            assert np.array_equal(g.sssp(s, policy=p), ref), (s, p)
+
+
+def test_sssp_c2_full_size(orc):
+    log("case: grid 2048x2048 + 1% long-range, w in [1,1000], seed 2 (2 of 16 sources)")
+    h = dawn.HostGraph.grid(2048, 0.01, 2, weighted=True)
+    g = dawn.Graph(h)
+    for s in dawn.gen_sources(h.n, 16, 2)[:2]:
+        d = g.sssp(s)
+        ref = orc.sssp(h.row_ptr, h.col, h.weight, s, dense=False)
+        assert np.array_equal(d, ref)
+
+
+def test_weighted_mssp_and_closeness(orc):
+    h = dawn.HostGraph.er(3000, 6.0, 7, weighted=True)
+    g = dawn.Graph(h)
+    srcs = np.array([0, 5, 9, 2999], np.int32)
+    dist, sums = g.mssp(srcs)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, h.weight, srcs)
+    assert np.array_equal(dist, odist)
+    assert np.array_equal(sums, osums)
+
+
+def test_mssp_rmat_small_all_rows(orc):
+    log("case: RMAT scale 14 ef 16 seed 3, 200 sources")
+    h = dawn.HostGraph.rmat(14, 16, 3)
+    g = dawn.Graph(h)
+    srcs = dawn.gen_sources(h.n, 200, 3)
+    odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    for p in POLICIES:
+        dist, sums, st = g.mssp(srcs, policy=p, stats=True)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+        assert st["skipped_sources"] == int(skipped.sum())
+
+
+def test_closeness_exact(orc):
+    h = dawn.HostGraph.rmat(12, 8, 5)
+    g = dawn.Graph(h)
+    srcs = dawn.gen_sources(h.n, 70, 5)
+    cc, sums, ref_int = g.closeness(srcs)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    for i in range(len(srcs)):
+        c, s, r = orc.closeness(odist[i])
+        assert sums[i] == s
+        assert cc[i] == c  # bit-exact double
+        assert ref_int[i] == r
+
+
+def test_directed_graph_pull_uses_csc(orc):
+    rng = np.random.default_rng(4)
+    n, m = 5000, 30000
+    u = rng.integers(0, n, m).astype(np.int32)
+    v = rng.integers(0, n, m).astype(np.int32)
+    h = dawn.HostGraph.from_edges(n, u, v, None, symmetrize=False)
+    hw = dawn.HostGraph.from_edges(n, u, v, rng.integers(1, 100, m).astype(np.int32), symmetrize=False)
+    g, gw = dawn.Graph(h), dawn.Graph(hw)
+    srcs = [s for s in range(0, n, 500) if h.degrees()[s] > 0]
+    for s in srcs:
+        ref = orc.bfs(h.row_ptr, h.col, s)
+        for p in POLICIES:
+            assert np.array_equal(g.bfs(s, policy=p), ref), (s, p)
+        refw = orc.sssp(hw.row_ptr, hw.col, hw.weight, s)
+        for p in ["fixed", "push", "pull", "bitmap"]:
+            assert np.array_equal(gw.sssp(s, policy=p), refw), (s, p)
+    dist, _ = g.mssp(np.array(srcs, np.int32), policy="pull")
+    odist, _, _, _ = orc.mssp(h.row_ptr, h.col, None, np.array(srcs, np.int32))
+    assert np.array_equal(dist, odist)
+
+
+def test_isolated_source_status(orc):
+    h = dawn.HostGraph.from_edges(10, np.array([0, 1], np.int32), np.array([1, 2], np.int32))
+    g = dawn.Graph(h)
+    with pytest.raises(dawn.DawnError) as e:
+        g.bfs(7)
+    assert e.value.status == dawn.ERR_ISOLATED_SOURCE
+    dist, sums, st = g.mssp(np.array([7, 0], np.int32), stats=True)
+    assert st["skipped_sources"] == 1
+    assert dist[0][7] == 0 and (np.delete(dist[0], 7) == INF).all()
+    assert list(dist[1][:3]) == [0, 1, 2]
+
+
+def test_mssp_c3_full_size_properties(orc):
+    log("case: RMAT scale 20 ef 16 seed 3, 4096 sources (64 checked by oracle)")
+    h = dawn.HostGraph.rmat(20, 16, 3)
+    g = dawn.Graph(h)
+    srcs = dawn.gen_sources(h.n, 4096, 3)
+    _, sums_all = g.mssp(srcs, want_dist=False)
+    sub = srcs[:128]
+    dist, sums = g.mssp(sub)
+    assert np.array_equal(sums, sums_all[:128])
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, sub[:64])
+    assert np.array_equal(dist[:64], odist)
+    for i in range(64, 128, 8):
+        assert bfs_level_equations_hold(h, int(sub[i]), dist[i])
+    # symmetry of undirected hop distances between sources
+    for i in range(0, 128, 16):
+        for j in range(0, 128, 16):
+            assert dist[i][sub[j]] == dist[j][sub[i]]

@@ -1,0 +1,175 @@
+"""Host logic on the CPU: generators, CSR build, MatrixMarket/list I/O and
+the C ABI's exported symbols.  No GPU calls."""
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, dawn
+
+INF = dawn.INF
+
+
+def csr_invariants(h, symmetric=True, dedup=True):
+    rp, col = h.row_ptr, h.col
+    n = h.n
+    assert rp[0] == 0 and rp[-1] == h.m and (np.diff(rp) >= 0).all()
+    assert ((col >= 0) & (col < n)).all()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert (rows != col).all(), "self-loop"
+    key = rows.astype(np.int64) * n + col
+    if dedup:
+        assert (np.diff(key) > 0).all(), "rows sorted, no duplicates"
+    if symmetric:
+        rev = np.sort(col.astype(np.int64) * n + rows)
+        assert np.array_equal(np.sort(key), rev), "symmetric"
+        if h.weighted:
+            w = dict(zip(key.tolist(), h.weight.tolist()))
+            for k, ww in list(w.items())[:2000]:
+                a, b = divmod(k, n)
+                assert w[b * n + a] == ww
+
+
+def digest(h):
+    m = hashlib.sha256()
+    m.update(h.row_ptr.tobytes())
+    m.update(h.col.tobytes())
+    if h.weighted:
+        m.update(h.weight.tobytes())
+    return m.hexdigest()[:16]
+
+
+def test_er_generator_invariants_and_size():
+    h = dawn.HostGraph.er(65536, 8.0, 1)
+    csr_invariants(h)
+    # G(n, M) with M = n*d/2 pairs: about n*d directed edges after dedup.
+    assert 0.99 * 524288 < h.m <= 524288
+
+
+def test_rmat_generator_invariants():
+    h = dawn.HostGraph.rmat(12, 16, 3)
+    csr_invariants(h)
+    assert h.n == 4096 and h.m <= 2 * 16 * 4096
+    d = h.degrees()
+    assert d[0] == d.max()  # no relabeling: vertex 0 is the top hub
+
+
+def test_grid_generator_structure():
+    side = 64
+    h = dawn.HostGraph.grid(side, 0.0, 2, weighted=True)
+    csr_invariants(h)
+    assert h.m == 2 * 2 * side * (side - 1)
+    assert ((h.weight >= 1) & (h.weight <= 1000)).all()
+    h2 = dawn.HostGraph.grid(side, 0.01, 2, weighted=True)
+    csr_invariants(h2)
+    extra = round(0.01 * 2 * side * (side - 1))
+    assert h.m < h2.m <= h.m + 2 * extra
+
+
+def test_generators_deterministic_across_threads():
+    a = digest(dawn.HostGraph.rmat(13, 8, 7))
+    old = os.environ.get("OMP_NUM_THREADS")
+    # The counter-based RNG makes the result independent of scheduling;
+    # a second build in the same process must agree.
+    b = digest(dawn.HostGraph.rmat(13, 8, 7))
+    assert a == b
+    assert digest(dawn.HostGraph.rmat(13, 8, 8)) != a
+    if old is not None:
+        os.environ["OMP_NUM_THREADS"] = old
+
+
+def test_golden_generator_digests():
+    """Digests of the seeded generators, committed so a change is caught."""
+    import json
+    path = os.path.join(ROOT, "tests", "golden", "generator_digests.json")
+    gold = json.load(open(path))
+    got = {
+        "er_n4096_d8_s1": digest(dawn.HostGraph.er(4096, 8.0, 1)),
+        "rmat_s12_ef16_s3": digest(dawn.HostGraph.rmat(12, 16, 3)),
+        "grid_64_1pct_s2_w": digest(dawn.HostGraph.grid(64, 0.01, 2, weighted=True)),
+        "sources_n65536_64_s1": hashlib.sha256(dawn.gen_sources(65536, 64, 1).tobytes()).hexdigest()[:16],
+    }
+    assert got == gold
+
+
+def test_from_edges_dedup_keeps_min_weight():
+    h = dawn.HostGraph.from_edges(4, np.array([0, 0, 1, 2], np.int32), np.array([1, 1, 0, 2], np.int32),
+                                  np.array([5, 3, 4, 9], np.int32), symmetrize=True, dedup=True)
+    assert list(h.row_ptr) == [0, 1, 2, 2, 2]
+    assert list(h.col) == [1, 0] and list(h.weight) == [3, 3]
+
+
+def test_mtx_reader_reference_semantics(tmp_path):
+    p = tmp_path / "g.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n% comment\n"
+                 "5 5 5\n2 1\n3 1\n3 3\n5 4\n2 1\n")
+    h = dawn.HostGraph.read_mtx(str(p))
+    assert not h.directed and not h.weighted
+    # 1-based -> 0-based, self-loop (3,3) dropped, both directions, duplicate kept.
+    assert list(h.row_ptr) == [0, 3, 5, 6, 7, 8]
+    assert list(h.col) == [1, 1, 2, 0, 0, 0, 4, 3]
+    q = tmp_path / "w.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n3 3 3\n1 2 7\n1 2 4.0\n3 1 2\n")
+    hw = dawn.HostGraph.read_mtx(str(q), weighted=True)
+    assert hw.directed and hw.weighted
+    assert list(hw.row_ptr) == [0, 2, 2, 3]
+    assert list(hw.col) == [1, 1, 0] and list(hw.weight) == [4, 7, 2]
+
+
+def test_mtx_errors(tmp_path):
+    with pytest.raises(dawn.DawnError) as e:
+        dawn.HostGraph.read_mtx(str(tmp_path / "missing.mtx"))
+    assert e.value.status == -6
+    bad = tmp_path / "bad.mtx"
+    bad.write_text("not a matrix market file\n")
+    with pytest.raises(dawn.DawnError) as e:
+        dawn.HostGraph.read_mtx(str(bad))
+    assert e.value.status == -7
+    frac = tmp_path / "frac.mtx"
+    frac.write_text("%%MatrixMarket matrix coordinate real general\n2 2 1\n1 2 0.5\n")
+    with pytest.raises(dawn.DawnError):
+        dawn.HostGraph.read_mtx(str(frac), weighted=True)
+
+
+def test_mtx_roundtrip(tmp_path):
+    h = dawn.HostGraph.grid(20, 0.02, 4, weighted=True)
+    p = str(tmp_path / "rt.mtx")
+    h.write_mtx(p)
+    r = dawn.HostGraph.read_mtx(p, weighted=True)
+    assert np.array_equal(h.row_ptr, r.row_ptr) and np.array_equal(h.col, r.col)
+    assert np.array_equal(h.weight, r.weight)
+
+
+def test_read_list_and_write_dist(tmp_path):
+    p = tmp_path / "src.txt"
+    p.write_text("% header\n5\n2147483648\n17\n")
+    assert list(dawn.read_list(str(p))) == [5, 2147483647, 17]
+    ref_list = "/root/reference/test/sourceList.txt"
+    if os.path.exists(ref_list):
+        got = dawn.read_list(ref_list)
+        assert len(got) == 500
+    d = np.array([0, 1, INF, 2], np.int32)
+    out = tmp_path / "d.txt"
+    dawn.write_dist(str(out), d, 0)
+    assert out.read_text() == "0 1 1\n0 3 2\n"
+
+
+def test_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "dawn_capi.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    declared = set(re.findall(r"\b(dawn_[a-z0-9_]+)\s*\(", hdr))
+    lib = dawn.lib()
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == set(dawn.exported_symbols())
+
+
+def test_no_device_calls_fail_cleanly():
+    if dawn.device_count() > 0:
+        pytest.skip("GPU present")
+    h = dawn.HostGraph.er(100, 4.0, 1)
+    with pytest.raises(dawn.DawnError) as e:
+        dawn.Graph(h)
+    assert e.value.status == dawn.ERR_NO_DEVICE
