@@ -90,7 +90,7 @@ struct MsWorkspace {
   DevBuf<unsigned long long> lane_stats;  // 3 * 64
   SchedState* h_state = nullptr;
   MsBatch* h_batch = nullptr;
-  std::map<long long, hipGraphExec_t> graphs;
+  std::map<std::pair<long long, const void*>, hipGraphExec_t> graphs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, e_a = nullptr, e_b = nullptr;
 };
 
@@ -577,7 +577,8 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   // the traversal finished within the chunk).  Sources and the output
   // pointer live in the device batch block, so one graph serves all batches.
   auto get_graph = [&](int want_edges, hipGraphExec_t* exec) -> int {
-    const long long key = ((long long)policy << 8) | want_edges;
+    const std::pair<long long, const void*> key{((long long)policy << 8) | want_edges,
+                                                (const void*)p.mlp};
     auto it = ws->graphs.find(key);
     if (it != ws->graphs.end()) {
       *exec = it->second;

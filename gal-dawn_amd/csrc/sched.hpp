@@ -38,7 +38,8 @@ struct SchedState {
   int32_t last_mode;
   int32_t levels;    // number of non-empty levels (set when done)
   int32_t force_mode;// >= 0: every step uses this mode
-  int32_t pad;
+  int32_t source;    // single-source engines: the traversal's source (read by
+                     // captured step kernels; never a baked launch argument)
 };
 
 // Constant per traversal.

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 
 #include "common.hpp"
@@ -31,7 +32,7 @@ namespace dawn {
 
 struct SsArgs {
   int32_t n;
-  int32_t source;
+  int32_t source;    // read only by the (uncaptured) seed kernels
   int64_t words;  // bitmap words per level slot
   const int32_t* row_ptr;
   const int32_t* col;
@@ -58,7 +59,7 @@ struct SsWorkspace {
   DevBuf<unsigned long long> reach;  // [2]
   SchedState* h_state = nullptr;     // pinned
   // Captured K-step chunks keyed by (kind, policy, K).
-  std::map<long long, hipGraphExec_t> graphs;
+  std::map<std::pair<long long, const void*>, hipGraphExec_t> graphs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -151,6 +152,7 @@ __global__ void bfs_seed_kernel(SsArgs a, int32_t policy) {
   st.e2c = a.row_ptr[a.n];
   st.scout = d;
   st.force_mode = -1;
+  st.source = s;
   st.last_mode = DAWN_MODE_PUSH;
   *a.st = st;
   (void)policy;
@@ -189,6 +191,7 @@ __global__ void sssp_seed_kernel(SsArgs a) {
   st.e2c = a.row_ptr[a.n];
   st.scout = (long long)c.scout;
   st.force_mode = -1;
+  st.source = s;
   st.last_mode = DAWN_MODE_PUSH;
   *a.st = st;
 }
@@ -341,9 +344,9 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
 
 // Relaxation of edge (u -> v, w) with du = current distance of u.  Returns
 // true when v joins the next frontier for the first time this level.
-__device__ __forceinline__ bool relax(const SsArgs& a, int32_t du, int32_t v,
+__device__ __forceinline__ bool relax(const SsArgs& a, int32_t src, int32_t du, int32_t v,
                                       int32_t w, uint32_t* bm_out) {
-  if (v == a.source) return false;
+  if (v == src) return false;
   const long long nd = (long long)du + (long long)w;
   if (nd >= kInf) return false;
   const int32_t ndi = (int32_t)nd;
@@ -364,6 +367,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
   __shared__ int s_scan[kBlock / kWave];
   const SchedState st = *a.st;
   if (st.done) return;
+  const int32_t src = st.source;
   const int lv = st.step;
   const int pi = (lv + 2) % 3, ci = lv % 3, ni = (lv + 1) % 3;
   const uint32_t* bm_in = a.bm + pi * a.words;
@@ -405,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
           int j = owner_of(s_off, e);
           int k = s_beg[j] + (e - s_off[j]);
           v = a.col[k];
-          disc = relax(a, s_du[j], v, a.w[k], bm_out);
+          disc = relax(a, src, s_du[j], v, a.w[k], bm_out);
         }
         emit(disc, v, q_out, qlen, acc, a.row_ptr);
       }
@@ -418,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
       const int32_t v = (int32_t)(base + lane);
-      const bool cand = v < a.n && v != a.source;
+      const bool cand = v < a.n && v != src;
       int beg = 0, deg = 0;
       int32_t cur = kInf;
       if (cand) {
@@ -484,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
         int32_t v = 0;
         if (light && j < deg) {
           v = a.col[beg + j];
-          disc = relax(a, du, v, a.w[beg + j], bm_out);
+          disc = relax(a, src, du, v, a.w[beg + j], bm_out);
         }
         emit(disc, v, q_out, qlen, acc, a.row_ptr);
       }
@@ -500,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
           int32_t v = 0;
           if (k < he) {
             v = a.col[k];
-            disc = relax(a, hdu, v, a.w[k], bm_out);
+            disc = relax(a, src, hdu, v, a.w[k], bm_out);
           }
           emit(disc, v, q_out, qlen, acc, a.row_ptr);
         }
@@ -599,9 +603,14 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
 
   const unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock, 4096);
   // A chunk = K decide+expand pairs, captured once per (kind, policy, mode, K).
-  const int K = g->n < (1 << 20) ? 8 : 16;
-  const long long key = ((long long)kind << 40) | ((long long)(policy & 0xff) << 32) |
-                        ((long long)(force_mode + 1) << 16) | K;
+  int K = g->n < (1 << 20) ? 8 : 16;
+  if (const char* e = std::getenv("DAWN_SS_CHUNK")) K = std::max(1, std::atoi(e));
+  // Launch arguments are baked into the graph: key it by every one that
+  // varies between calls (the network pointer changes when one is installed).
+  const std::pair<long long, const void*> key{
+      ((long long)kind << 40) | ((long long)(policy & 0xff) << 32) |
+          ((long long)(force_mode + 1) << 16) | K,
+      (const void*)p.mlp};
   hipGraphExec_t exec = nullptr;
   auto it = ws->graphs.find(key);
   if (it != ws->graphs.end()) {
