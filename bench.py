@@ -1,0 +1,237 @@
+"""Benchmark: unweighted MSSP on RMAT scale 24, edge factor 16 (symmetrized,
+deduplicated, seed 99), 4096 uniform sources (seed 99), on one MI355X.
+
+A step = one MSSP pass over all sources (64 bit-packed batches of 64).
+Reported value: GTEPS of the learned schedule, whole job, inputs resident
+in HBM.  Also: sources/s, the fixed alpha/beta schedule's numbers, the
+per-step oracle-best schedule and regret, a bit-exact spot check against
+the CPU oracle, the dominant kernel's roofline, and the reference's own CPU
+MSSP (oracle/_ref) timed on a bounded sample of the same workload.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__  # noqa: E402
+
+METRIC = "GTEPS + MSSP sources/s on 1 MI355X, learned vs fixed schedule, bit-exact dists"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_bytes(n, m, levels):
+    """Algorithmic HBM bytes of one 64-source batch: offsets and columns
+    once, one 8-byte frontier word per directed edge, read+write of one
+    8-byte word per vertex per step, 64 int32 distances per vertex."""
+    return 4 * (n + 1) + 12 * m + 16 * n * levels + 256 * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--sources", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=99)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="sources for the CPU baseline (0: auto)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--check", type=int, default=16, help="sources checked bit-exactly vs the oracle")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(sources shard with no communication; see DESIGN.md)"}))
+        sys.exit(2)
+
+    dawn = __graft_entry__.load_package()
+    if dawn.device_count() < 1:
+        print(json.dumps({"error": "no HIP device visible"}))
+        sys.exit(1)
+
+    t0 = time.time()
+    h = dawn.HostGraph.rmat(args.scale, args.edge_factor, args.seed)
+    build_s = time.time() - t0
+    n, m = h.n, h.m
+    log(f"graph RMAT scale {args.scale} ef {args.edge_factor}: n={n} m={m} built in {build_s:.1f}s")
+    t0 = time.time()
+    g = dawn.Graph(h)
+    upload_s = time.time() - t0
+    src = dawn.gen_sources(n, args.sources, args.seed)
+
+    # Learned scheduler network (trained offline by gal-dawn_amd/train_sched.py).
+    learned_loaded = False
+    wpath = os.path.join(ROOT, "gal-dawn_amd", "sched_mlp.json")
+    if os.path.exists(wpath):
+        w = json.load(open(wpath))
+        dawn.sched_set_mlp(np.array(w["params"], np.float32), np.array(w["mean"], np.float32),
+                           np.array(w["inv_std"], np.float32))
+        learned_loaded = True
+    log(f"learned network loaded: {learned_loaded}")
+
+    results = {}
+    sums_by_policy = {}
+    for pol in ("learned", "fixed"):
+        for _ in range(args.warmup):
+            g.mssp(src, policy=pol, want_dist=False)
+        t0 = time.time()
+        dev_ms = 0.0
+        stats = None
+        for _ in range(args.steps):
+            _, sums, st = g.mssp(src, policy=pol, want_dist=False, stats=True)
+            dev_ms += st["device_ms"]
+            stats = st
+        wall = time.time() - t0
+        sums_by_policy[pol] = sums
+        t = wall / args.steps
+        results[pol] = {
+            "ms_per_step": 1e3 * t,
+            "device_ms_per_step": dev_ms / args.steps,
+            "gteps": stats["edges_reached"] / t / 1e9,
+            "sources_per_s": args.sources / t,
+            "steps_per_pass": stats["steps"],
+            "mode_steps": stats["mode_steps"],
+            "edges_reached": stats["edges_reached"],
+        }
+        log(f"{pol}: {results[pol]}")
+    assert np.array_equal(sums_by_policy["learned"], sums_by_policy["fixed"])
+
+    # Per-step oracle-best schedule on the first 4 batches: every mode timed
+    # from identical state (dawn_sched_trace), then each policy's choices.
+    sub = src[:256]
+    regret = {}
+    for pol in ("fixed", "learned"):
+        modes, feat, times = g.sched_trace(sub, policy=pol)
+        regret[pol] = float(times[np.arange(len(modes)), modes].sum())
+        regret["oracle_best"] = float(times.min(axis=1).sum())
+        regret[f"{pol}_modes"] = modes.tolist()
+    sched = {
+        "subset_sources": int(len(sub)),
+        "oracle_best_ms": regret["oracle_best"],
+        "fixed_ms": regret["fixed"],
+        "learned_ms": regret["learned"],
+        "learned_regret": regret["learned"] / regret["oracle_best"] - 1.0,
+        "fixed_regret": regret["fixed"] / regret["oracle_best"] - 1.0,
+        "learned_speedup_vs_fixed": results["fixed"]["ms_per_step"] / results["learned"]["ms_per_step"],
+        "learned_network": learned_loaded,
+    }
+    log(f"schedule: {sched}")
+
+    # Profiling pass (same kernels, one launch at a time with HIP events on
+    # the engine's stream) over 8 batches: per-kernel average durations.
+    prof, levels = g.mssp_profile(src[:512], policy="learned")
+    kern = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": v[0] / v[1] if v[1] else 0.0}
+            for k, v in prof.items()}
+    L = float(levels.mean())
+    total_prof = sum(v["total_ms"] for v in kern.values())
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    # Algorithmic bytes per launch of each kernel.
+    per_launch = {
+        "final": 8 * n * (L + 1) + 256 * n,           # seen + L level words in, 64 int32 out
+        "finalize": 24 * n,                           # next + seen in, level word out
+        "expand_pull": (4 * (n + 1) + 12 * m) / L + 16 * n,  # the batch's edge traffic per step
+        "expand_push": (4 * (n + 1) + 12 * m) / L + 16 * n,
+        "expand_bitmap": (4 * (n + 1) + 12 * m) / L + 16 * n,
+        "fill": 24 * n,
+    }
+    dom_bytes = per_launch.get(dom, 0.0)
+    dom_avg_s = kern[dom]["avg_ms"] / 1e3
+    achieved = dom_bytes / dom_avg_s / 1e9 if dom_avg_s > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "avg_launch_ms": kern[dom]["avg_ms"],
+        "bytes_per_launch": dom_bytes,
+    }
+    b_batch = batch_bytes(n, m, L)
+    lt = results["learned"]["device_ms_per_step"] / 1e3
+    batch_frac = (b_batch * (args.sources / 64)) / lt / 1e9 / HBM_PEAK_GBS
+    log(f"kernels: {kern}; levels/batch {L:.2f}; dominant {dom}")
+
+    # Bit-exact spot check against the CPU oracle on sampled sources.
+    orc = __graft_entry__.load_oracle()
+    chk = src[np.linspace(0, args.sources - 1, args.check).astype(int)]
+    dist, sums = g.mssp(chk, policy="learned")
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, chk, threads=min(16, os.cpu_count() or 1))
+    parity = bool(np.array_equal(dist, odist) and np.array_equal(sums, osums))
+    pos = [int(np.flatnonzero(src == c)[0]) for c in chk]
+    parity_sums = bool(np.array_equal(sums_by_policy["learned"][pos], osums))
+    log(f"parity: dist {parity}, timed-run sums {parity_sums}")
+
+    cpu = None
+    if not args.skip_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        sample = args.cpu_sample or 2 * threads
+        csrc = src[:sample]
+        if orc.ref_available():
+            t, rep = orc.ref_mssp_time(h.row_ptr, h.col, csrc, threads)
+            kind = "reference"
+        else:
+            _, _, _, t = orc.mssp(h.row_ptr, h.col, None, csrc, want_dist=False, threads=threads)
+            kind = "port"
+        cpu = {
+            "value": sample / t,
+            "unit": "sources/s",
+            "cores": threads,
+            "kind": kind,
+            "sample": f"{sample} of the {args.sources} sources, DAWN::MSSP_CPU::run (one source per "
+                      f"OpenMP thread, sequential sparse BFS), wall {t:.2f}s",
+            # GTEPS-equivalent: the average reached edges per source of this workload.
+            "gteps": results["learned"]["edges_reached"] / args.sources * sample / t / 1e9,
+        }
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": results["learned"]["gteps"],
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": results["learned"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"RMAT a=.57 b=.19 c=.19 scale {args.scale} ef {args.edge_factor} symmetrized "
+                        f"dedup seed {args.seed}; unweighted MSSP of {args.sources} sources (64-bit packed)",
+            "n": n, "m": m, "sources": args.sources, "parallelism": "single GPU",
+        },
+        "sources_per_s": results["learned"]["sources_per_s"],
+        "fixed": results["fixed"],
+        "learned": results["learned"],
+        "schedule": sched,
+        "roofline": roofline,
+        "batch_roofline_frac": batch_frac,
+        "kernels": kern,
+        "levels_per_batch": L,
+        "parity": {"checked_sources": int(len(chk)), "dist_bit_exact": parity,
+                   "timed_sums_bit_exact": parity_sums},
+        "graph_build_s": build_s,
+        "upload_s": upload_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

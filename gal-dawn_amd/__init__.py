@@ -119,6 +119,7 @@ def _sig(lib):
         "dawn_apsp_block": (c.c_int, [P, c.c_int32, c.c_int32, P, _I64P, c.c_int32, c.POINTER(Stats)]),
         "dawn_closeness": (c.c_int, [P, _I32P, c.c_int32, _F64P, _I64P, _F32P, c.c_int32, c.POINTER(Stats)]),
         "dawn_batch_mssp": (c.c_int, [c.POINTER(P), c.c_int32, _I32P, c.c_int32, P, _I64P, c.POINTER(Stats)]),
+        "dawn_mssp_profile": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F64P, _I64P, _I32P]),
         "dawn_sched_set_mlp": (c.c_int, [_F32P, _F32P, _F32P]),
         "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, c.c_int32, _I32P]),
         "dawn_sched_trace": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _I32P, _F32P, _F32P, c.c_int32, _I32P]),
@@ -414,6 +415,20 @@ class Graph:
                                     _ptr(sums, ctypes.c_int64), _ptr(ref, ctypes.c_float),
                                     _policy(policy), ctypes.byref(st)), "dawn_closeness")
         return cc, sums, ref
+
+    PROF_NAMES = ("fill", "seed", "decide", "expand_push", "expand_pull", "expand_bitmap",
+                  "finalize", "final")
+
+    def mssp_profile(self, sources, policy="fixed"):
+        """Per-kernel HIP-event times of an unweighted MSSP (profiling pass)."""
+        src = _i32(sources)
+        ms = np.zeros(8, np.float64)
+        cnt = np.zeros(8, np.int64)
+        levels = np.zeros((len(src) + 63) // 64, np.int32)
+        _check(lib().dawn_mssp_profile(self._h, _ptr(src, ctypes.c_int32), len(src), _policy(policy),
+                                       _ptr(ms, ctypes.c_double), _ptr(cnt, ctypes.c_int64),
+                                       _ptr(levels, ctypes.c_int32)), "dawn_mssp_profile")
+        return ({k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PROF_NAMES)}, levels)
 
     def sched_record(self, sources, repeats: int = 3, max_rows: int = 4096):
         src = _i32(sources)

@@ -21,6 +21,8 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, i
 int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
                   float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
                   int32_t* rows_out, int32_t policy);
+int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
+                   double* kernel_ms, int64_t* launches, int32_t* levels_out);
 
 // ---- scheduler network store (one copy per device) -------------------------
 namespace {
@@ -164,6 +166,15 @@ int dawn_closeness(const dawn_graph* gc, const int32_t* sources, int32_t count,
     }
   }
   return DAWN_OK;
+}
+
+int dawn_mssp_profile(const dawn_graph* gc, const int32_t* sources, int32_t count,
+                      int32_t policy, double* kernel_ms, int64_t* launches,
+                      int32_t* levels) {
+  dawn_graph* g = const_cast<dawn_graph*>(gc);
+  if (!g) return DAWN_ERR_INVALID_ARG;
+  if (g->weighted) return DAWN_ERR_UNSUPPORTED;
+  return run_ms_profile(g, sources, count, policy, kernel_ms, launches, levels);
 }
 
 int dawn_sched_set_mlp(const float* params, const float* feat_mean, const float* feat_inv_std) {

@@ -758,3 +758,76 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
 }
 
 }  // namespace dawn
+
+namespace dawn {
+
+// Same kernels as run_msbfs, launched one by one with HIP events around each
+// launch on the graph's stream.  kernel_ms / launches are indexed by
+// DAWN_PROF_* (fill, seed, decide, expand push, expand pull, expand bitmap,
+// finalize, final).  levels_out: per batch number of levels (may be null).
+int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
+                   double* kernel_ms, int64_t* launches, int32_t* levels_out) {
+  if (!g || S <= 0 || !sources || !kernel_ms || !launches) return DAWN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  DAWN_TRY(ensure_ms(g));
+  MsWorkspace* ws = g->ms;
+  DAWN_TRY(ws->distbuf.alloc((size_t)kLanes * g->n));
+  hipStream_t s = g->stream;
+  MsArgs a = make_args(g);
+  SchedParams p = make_params(g, policy);
+  SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
+  const unsigned grid = ms_grid(g);
+  unsigned long long* ls = ws->lane_stats.p;
+  MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 0};
+  for (int i = 0; i < 8; i++) {
+    kernel_ms[i] = 0;
+    launches[i] = 0;
+  }
+  auto timed = [&](int slot, auto&& launch) -> int {
+    DAWN_HIP(hipEventRecord(ws->e_a, s));
+    launch();
+    DAWN_HIP(hipGetLastError());
+    DAWN_HIP(hipEventRecord(ws->e_b, s));
+    DAWN_HIP(hipEventSynchronize(ws->e_b));
+    float ms = 0.f;
+    DAWN_HIP(hipEventElapsedTime(&ms, ws->e_a, ws->e_b));
+    kernel_ms[slot] += ms;
+    launches[slot] += 1;
+    return DAWN_OK;
+  };
+  int batch = 0;
+  for (int32_t b0 = 0; b0 < S; b0 += kLanes, batch++) {
+    const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
+    stage_batch(ws, sources + b0, lanes);
+    ws->h_batch->dist = ws->distbuf.p;
+    ws->h_batch->stride = g->n;
+    DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+    DAWN_TRY(timed(0, [&] { ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o); }));
+    DAWN_TRY(timed(1, [&] { ms_seed_kernel<<<1, 64, 0, s>>>(a); }));
+    int lo = 0;
+    for (int64_t guard_steps = 0;; guard_steps++) {
+      if (guard_steps > (int64_t)g->n + 2) return DAWN_ERR_HIP;
+      DAWN_TRY(timed(2, [&] { ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr); }));
+      DAWN_HIP(hipMemcpy(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost));
+      const SchedState st = *ws->h_state;
+      if (st.done) {
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, st.levels, 1); }));
+        if (levels_out) levels_out[batch] = st.levels;
+        break;
+      }
+      const int slot = st.mode == DAWN_MODE_PUSH ? 3 : st.mode == DAWN_MODE_PULL ? 4 : 5;
+      DAWN_TRY(timed(slot, [&] { ms_expand_kernel<<<grid, kBlock, 0, s>>>(a); }));
+      if (st.mode != DAWN_MODE_PULL)
+        DAWN_TRY(timed(6, [&] { ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a); }));
+      if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
+        lo = st.step;
+      }
+    }
+  }
+  DAWN_HIP(hipStreamSynchronize(s));
+  return DAWN_OK;
+}
+
+}  // namespace dawn

@@ -189,6 +189,19 @@ int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
                     int32_t* dist, const int64_t* dist_offsets,
                     dawn_stats* stats);
 
+/* Profiling pass of dawn_mssp (unweighted): the same kernels launched one
+ * at a time with HIP events around each launch on the graph's stream.
+ * kernel_ms[8] / launches[8] are indexed by DAWN_PROF_*; levels receives
+ * the level count of each 64-source batch (ceil(count/64) entries). */
+enum {
+  DAWN_PROF_FILL = 0, DAWN_PROF_SEED = 1, DAWN_PROF_DECIDE = 2,
+  DAWN_PROF_EXPAND_PUSH = 3, DAWN_PROF_EXPAND_PULL = 4,
+  DAWN_PROF_EXPAND_BITMAP = 5, DAWN_PROF_FINALIZE = 6, DAWN_PROF_FINAL = 7
+};
+int dawn_mssp_profile(const dawn_graph* g, const int32_t* sources,
+                      int32_t count, int32_t policy, double* kernel_ms,
+                      int64_t* launches, int32_t* levels);
+
 /* ----------------------------------------------------------- scheduler --- */
 
 /* Number of per-step features read by the scheduler MLP. */
