@@ -18,6 +18,8 @@ namespace dawn {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;          // threads per workgroup for step kernels
 constexpr int kHeavyDegree = 32;     // rows >= this are expanded by a whole wave
+constexpr int kPullChunk = 256;      // in-edges per heavy pull chunk
+constexpr int kPushChunk = 256;      // out-edges per push queue entry
 constexpr int32_t kInf = DAWN_INF;
 
 // Records the last HIP error text (per thread) for diagnostics.
@@ -90,6 +92,13 @@ struct dawn_graph {
   int32_t* in_row_ptr = nullptr;  // CSC (directed only; else == row_ptr)
   int32_t* in_col = nullptr;
   int32_t* in_weight = nullptr;
+  // In-edge lists of vertices with in-degree >= kHeavyDegree, cut into
+  // chunks of at most kPullChunk edges: chunk c covers in_col[beg[c], end[c])
+  // of vertex v[c].  One wave per chunk in the pull step.
+  int32_t* hc_v = nullptr;
+  int32_t* hc_beg = nullptr;
+  int32_t* hc_end = nullptr;
+  int64_t num_hc = 0;
   std::vector<int32_t> h_row_ptr; // host copy of offsets (degrees for drivers)
   hipStream_t stream = nullptr;
   std::mutex lock;                // one traversal per graph at a time
@@ -118,6 +127,25 @@ __device__ __forceinline__ int wave_append(unsigned int* counter, bool pred) {
   if (lane_id() == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
   base = __shfl(base, leader);
   return pred ? (int)(base + (unsigned int)__popcll(m & lanes_below_mask())) : -1;
+}
+
+// Wave-aggregated append of k >= 0 slots per lane; returns the lane's first
+// slot.  Must be called by all 64 lanes.
+__device__ __forceinline__ int wave_append_n(unsigned int* counter, int k) {
+  const int l = lane_id();
+  int inc = k;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(inc, o);
+    if (l >= o) inc += y;
+  }
+  const int total = __shfl(inc, 63);
+  unsigned int base = 0;
+  if (total > 0) {
+    if (l == 0) base = atomicAdd(counter, (unsigned int)total);
+    base = __shfl(base, 0);
+  }
+  return (int)base + inc - k;
 }
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {

@@ -112,6 +112,7 @@ int dawn_graph_create(const dawn_csr* csr, dawn_graph** out) {
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(DAWN_ERR_HIP);
   int rc;
+  std::vector<int32_t> in_rp_host;  // CSC offsets (directed graphs)
   if ((rc = up(&g->row_ptr, csr->row_ptr, (size_t)n + 1))) return fail(rc);
   if ((rc = up(&g->col, csr->col, (size_t)m))) return fail(rc);
   if (csr->weight && (rc = up(&g->weight, csr->weight, (size_t)m))) return fail(rc);
@@ -127,9 +128,28 @@ int dawn_graph_create(const dawn_csr* csr, dawn_graph** out) {
     int32_t maxi = 0;
     if ((rc = build_csc(csr, ip, ic, iw, &maxi))) return fail(rc);
     g->max_in_degree = maxi;
+    in_rp_host = ip;
     if ((rc = up(&g->in_row_ptr, ip.data(), (size_t)n + 1))) return fail(rc);
     if ((rc = up(&g->in_col, ic.data(), (size_t)m))) return fail(rc);
     if (csr->weight && (rc = up(&g->in_weight, iw.data(), (size_t)m))) return fail(rc);
+  }
+  {
+    // Heavy in-edge chunks for the pull step (see dawn_graph::hc_v).
+    const std::vector<int32_t>& irp = g->directed ? in_rp_host : g->h_row_ptr;
+    std::vector<int32_t> hv, hb, he;
+    for (int32_t v = 0; v < n; v++) {
+      const int32_t b = irp[v], e = irp[v + 1];
+      if (e - b < kHeavyDegree) continue;
+      for (int32_t k = b; k < e; k += kPullChunk) {
+        hv.push_back(v);
+        hb.push_back(k);
+        he.push_back(std::min(e, k + kPullChunk));
+      }
+    }
+    g->num_hc = (int64_t)hv.size();
+    if ((rc = up(&g->hc_v, hv.data(), hv.size()))) return fail(rc);
+    if ((rc = up(&g->hc_beg, hb.data(), hb.size()))) return fail(rc);
+    if ((rc = up(&g->hc_end, he.data(), he.size()))) return fail(rc);
   }
   *out = g;
   return DAWN_OK;
@@ -146,6 +166,9 @@ int dawn_graph_destroy(dawn_graph* g) {
     if (g->in_col) (void)hipFree(g->in_col);
     if (g->in_weight) (void)hipFree(g->in_weight);
   }
+  if (g->hc_v) (void)hipFree(g->hc_v);
+  if (g->hc_beg) (void)hipFree(g->hc_beg);
+  if (g->hc_end) (void)hipFree(g->hc_end);
   if (g->row_ptr) (void)hipFree(g->row_ptr);
   if (g->col) (void)hipFree(g->col);
   if (g->weight) (void)hipFree(g->weight);

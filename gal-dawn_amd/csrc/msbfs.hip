@@ -12,17 +12,20 @@
 //                             between steps by the finalize pass)
 //   hist   u64[H][npad]       level history: slot l%H holds the lanes that
 //                             reached v exactly at level l (= frontier of l)
-//   q      int32[2][n]        frontier vertex queue (any lane), level parity
+//   q, qb  int32[2][qcap]     frontier queue (level parity): one entry per
+//                             256 out-edges of a frontier vertex (vertex,
+//                             first edge), so push work per entry is bounded
 //   dist   int32[64][n]       output rows of the batch (caller or workspace)
-// Step modes (all produce the same next level):
+// Step modes (all produce the same next level in next[]):
 //   PULL    every vertex missing a lane ORs the previous level's words of its
-//           in-neighbours (CSC), early exit once all missing lanes are found;
-//           writes the level slot, seen and the queue directly.
-//   PUSH    queue of frontier vertices, workgroup-balanced expansion,
+//           in-neighbours (CSC): light vertices one lane each with early exit
+//           once all missing lanes are found; heavy vertices' in-edge lists
+//           in 256-edge chunks, one wave per chunk, merged by atomicOr.
+//   PUSH    queue entries, workgroup-balanced expansion (LDS degree scan),
 //           64-bit atomicOr of (frontier & ~seen[v]) into next[v].
 //   BITMAP  dense scan of the previous level slot, same scatter as PUSH.
-//   PUSH/BITMAP are followed by a finalize pass that folds next into the
-//   level slot and seen.
+//   Every step ends with a finalize pass that folds next into the level
+//   slot, seen, the queue and the level counters.
 // After the last level one pass turns the history into int32 distances
 // (64 coalesced row stores per wave) and per-lane sums/counts (64x64 bit
 // transposes + popcount), so no distance is written twice.
@@ -64,7 +67,13 @@ struct MsArgs {
   uint64_t* seen;
   uint64_t* next;
   uint64_t* hist;
-  int32_t* q;
+  int32_t* q;          // [2][qcap] frontier entries: vertex
+  int32_t* qb;         // [2][qcap] first out-edge of the entry's chunk
+  int64_t qcap;
+  const int32_t* hc_v;    // heavy in-edge chunks (pull)
+  const int32_t* hc_beg;
+  const int32_t* hc_end;
+  int64_t num_hc;
   LevelCounters* ring;
   SchedState* st;
   MsBatch* batch;
@@ -81,7 +90,8 @@ struct MsWorkspace {
   int32_t n = 0;
   int64_t npad = 0;
   DevBuf<uint64_t> seen, next, hist, seen_bak;
-  DevBuf<int32_t> q, distbuf;
+  DevBuf<int32_t> q, qb, distbuf;
+  int64_t qcap = 0;
   DevBuf<LevelCounters> ring, ring_bak;
   DevBuf<SchedState> st;
   DevBuf<MsBatch> batch;
@@ -146,16 +156,24 @@ struct MsAcc {
 // wave-aggregated); lanes with v >= n pass valid = false.
 __device__ __forceinline__ void ms_commit(const MsArgs& a, bool valid, int32_t v,
                                           uint64_t sv, uint64_t nw, uint64_t* slot_out,
-                                          int32_t* q_out, unsigned* qlen, MsAcc& acc) {
+                                          int32_t* q_out, int32_t* qb_out, unsigned* qlen,
+                                          MsAcc& acc) {
   if (valid) {
     slot_out[v] = nw;
     if (nw) a.seen[v] = sv | nw;
   }
   const bool isf = valid && nw != 0;
-  int pos = wave_append(qlen, isf);
+  // A frontier vertex takes one queue entry per kPushChunk out-edges, so a
+  // push step's work per entry is bounded whatever the degree.
+  const int d = isf ? out_deg(a, v) : 0;
+  const int k = (d + kPushChunk - 1) / kPushChunk;
+  const int pos = wave_append_n(qlen, k);
   if (isf) {
-    q_out[pos] = v;
-    int d = out_deg(a, v);
+    const int32_t rb = a.row_ptr[v];
+    for (int i = 0; i < k; i++) {
+      q_out[pos + i] = v;
+      qb_out[pos + i] = rb + i * kPushChunk;
+    }
     acc.fv += 1;
     acc.fe += (unsigned)d;
     acc.scout += (unsigned)(d > 0 ? d : 1);
@@ -194,8 +212,12 @@ __global__ void ms_seed_kernel(MsArgs a) {
     const int32_t v = b.src[i];
     const uint64_t bit = 1ull << i;
     if (a.hist[v] == 0) {
-      a.q[c.qlen++] = v;
-      int d = out_deg(a, v);
+      const int d = out_deg(a, v);
+      for (int k = 0; k < d; k += kPushChunk) {
+        a.q[c.qlen] = v;
+        a.qb[c.qlen] = a.row_ptr[v] + k;
+        c.qlen++;
+      }
       c.fv += 1;
       c.fe += (unsigned)d;
       c.scout += (unsigned)d;
@@ -207,9 +229,10 @@ __global__ void ms_seed_kernel(MsArgs a) {
   // Vertices still missing a lane after level 0, and their in-degree sum.
   c.uv = (unsigned long long)a.n;
   c.ue = (unsigned long long)a.m;
-  for (unsigned i = 0; i < c.qlen; i++) {
-    const int32_t v = a.q[i];
-    if (a.seen[v] == ~0ull) {
+  for (int i = 0; i < b.nlanes; i++) {
+    const int32_t v = b.src[i];
+    const bool first = (a.hist[v] & ((1ull << i) - 1)) == 0;  // lowest lane of v
+    if (first && a.seen[v] == ~0ull) {
       c.uv -= 1;
       c.ue -= (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
     }
@@ -243,7 +266,8 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
 
   if (st.mode == DAWN_MODE_PUSH) {
-    const int32_t* q_in = a.q + (int64_t)((lv - 1) & 1) * a.n;
+    const int32_t* q_in = a.q + (int64_t)((lv - 1) & 1) * a.qcap;
+    const int32_t* qb_in = a.qb + (int64_t)((lv - 1) & 1) * a.qcap;
     const unsigned qn = a.ring[(lv - 1) % kRing].qlen;
     for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
       const int64_t i = c0 + threadIdx.x;
@@ -251,8 +275,8 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       uint64_t f = 0;
       if (i < qn) {
         const int32_t u = q_in[i];
-        beg = a.row_ptr[u];
-        deg = a.row_ptr[u + 1] - beg;
+        beg = qb_in[i];
+        deg = min(a.row_ptr[u + 1] - beg, kPushChunk);
         f = prev[u];
       }
       int total;
@@ -305,60 +329,50 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       }
     }
   } else {  // DAWN_MODE_PULL
-    int32_t* q_out = a.q + (int64_t)(lv & 1) * a.n;
-    LevelCounters* c_out = a.ring + (lv % kRing);
-    MsAcc acc;
+    // Light vertices (in-degree < kHeavyDegree): one lane each, early exit
+    // once every missing lane is found; the lane owns next[v].
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.npad;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
       const int32_t v = (int32_t)(base + lane);
-      const bool valid = v < a.n;
-      const uint64_t sv = valid ? a.seen[v] : ~0ull;
-      const uint64_t miss = ~sv;
-      int beg = 0, deg = 0;
-      if (miss) {
-        beg = a.in_row_ptr[v];
-        deg = a.in_row_ptr[v + 1] - beg;
-      }
+      const uint64_t miss = v < a.n ? ~a.seen[v] : 0ull;
+      if (!miss) continue;
+      const int beg = a.in_row_ptr[v];
+      const int deg = a.in_row_ptr[v + 1] - beg;
+      if (deg >= kHeavyDegree || deg == 0) continue;
       uint64_t acc_w = 0;
-      const bool light = miss && deg < kHeavyDegree;
-      if (light) {
-        for (int j = 0; j < deg; j++) {
-          acc_w |= prev[a.in_col[beg + j]];
-          if ((acc_w & miss) == miss) break;
-        }
+      for (int j = 0; j < deg; j++) {
+        acc_w |= prev[a.in_col[beg + j]];
+        if ((acc_w & miss) == miss) break;
       }
-      uint64_t hm = __ballot(miss && deg >= kHeavyDegree);
-      while (hm) {
-        const int l = __ffsll((unsigned long long)hm) - 1;
-        hm &= hm - 1;
-        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
-        const uint64_t hmiss = ((uint64_t)__shfl((unsigned)(miss >> 32), l) << 32) |
-                               (unsigned)__shfl((unsigned)(miss & 0xffffffffu), l);
-        uint64_t part = 0;
-        for (int k0 = hb; k0 < he; k0 += kWave) {
-          const int k = k0 + lane;
-          if (k < he) part |= prev[a.in_col[k]];
-          const uint64_t all = wave_or_u64(part);
-          if ((all & hmiss) == hmiss) break;
-        }
-        part = wave_or_u64(part);
-        if (lane == l) acc_w = part;
-      }
-      const uint64_t nw = acc_w & miss;
-      ms_commit(a, valid, v, sv, nw, cur, q_out, &c_out->qlen, acc);
+      if (acc_w & miss) a.next[v] = acc_w & miss;
     }
-    acc.flush(c_out);
+    // Heavy vertices: their in-edge lists are cut into chunks of at most
+    // kPullChunk edges, one wave per chunk, merged with atomicOr.
+    const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t c = wave; c < a.num_hc; c += nwaves) {
+      const int32_t v = a.hc_v[c];
+      const uint64_t miss = ~a.seen[v];
+      if (!miss) continue;
+      if ((a.next[v] & miss) == miss) continue;  // other chunks already found them all
+      const int he = a.hc_end[c];
+      uint64_t part = 0;
+      for (int k = a.hc_beg[c] + lane; k < he; k += kWave) part |= prev[a.in_col[k]];
+      part = wave_or_u64(part) & miss;
+      if (lane == 0 && part) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)part);
+    }
   }
 }
 
-// Folds next[] into the level slot after PUSH / BITMAP.
+// Folds next[] into the level slot, seen, the queue and the counters.
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   const SchedState st = *a.st;
-  if (st.done || st.mode == DAWN_MODE_PULL) return;
+  if (st.done) return;
   const int lv = st.step;
   uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad;
-  int32_t* q_out = a.q + (int64_t)(lv & 1) * a.n;
+  int32_t* q_out = a.q + (int64_t)(lv & 1) * a.qcap;
+  int32_t* qb_out = a.qb + (int64_t)(lv & 1) * a.qcap;
   LevelCounters* c_out = a.ring + (lv % kRing);
   MsAcc acc;
   const int64_t wstride = (int64_t)gridDim.x * kBlock;
@@ -373,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       if (nx) a.next[v] = 0ull;
       nw = nx & ~sv;
     }
-    ms_commit(a, valid, v, sv, nw, cur, q_out, &c_out->qlen, acc);
+    ms_commit(a, valid, v, sv, nw, cur, q_out, qb_out, &c_out->qlen, acc);
   }
   acc.flush(c_out);
 }
@@ -480,7 +494,11 @@ int ensure_ms(dawn_graph* g) {
   DAWN_TRY(ws->seen.alloc(ws->npad));
   DAWN_TRY(ws->next.alloc(ws->npad));
   DAWN_TRY(ws->hist.alloc((size_t)kHist * ws->npad));
-  DAWN_TRY(ws->q.alloc((size_t)2 * std::max(g->n, 1)));
+  // Queue entries per level: one per kPushChunk out-edges of each frontier
+  // vertex, at most n + m / kPushChunk.
+  ws->qcap = (int64_t)g->n + g->m / kPushChunk + 64;
+  DAWN_TRY(ws->q.alloc((size_t)(2 * ws->qcap)));
+  DAWN_TRY(ws->qb.alloc((size_t)(2 * ws->qcap)));
   DAWN_TRY(ws->ring.alloc(kRing));
   DAWN_TRY(ws->ring_bak.alloc(kRing));
   DAWN_TRY(ws->st.alloc(1));
@@ -511,6 +529,12 @@ MsArgs make_args(dawn_graph* g) {
   a.next = ws->next.p;
   a.hist = ws->hist.p;
   a.q = ws->q.p;
+  a.qb = ws->qb.p;
+  a.qcap = ws->qcap;
+  a.hc_v = g->hc_v;
+  a.hc_beg = g->hc_beg;
+  a.hc_end = g->hc_end;
+  a.num_hc = g->num_hc;
   a.ring = ws->ring.p;
   a.st = ws->st.p;
   a.batch = ws->batch.p;
@@ -818,8 +842,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
       }
       const int slot = st.mode == DAWN_MODE_PUSH ? 3 : st.mode == DAWN_MODE_PULL ? 4 : 5;
       DAWN_TRY(timed(slot, [&] { ms_expand_kernel<<<grid, kBlock, 0, s>>>(a); }));
-      if (st.mode != DAWN_MODE_PULL)
-        DAWN_TRY(timed(6, [&] { ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a); }));
+      DAWN_TRY(timed(6, [&] { ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a); }));
       if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
         DAWN_TRY(timed(7, [&] { ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
         lo = st.step;
