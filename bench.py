@@ -117,6 +117,11 @@ def main():
         regret[pol] = float(times[np.arange(len(modes)), modes].sum())
         regret["oracle_best"] = float(times.min(axis=1).sum())
         regret[f"{pol}_modes"] = modes.tolist()
+        if pol == "fixed":
+            for i in range(len(modes)):
+                log(f"step {i}: fv=2^{feat[i][0]:.1f} fe=2^{feat[i][1]:.1f} uv=2^{feat[i][2]:.1f} "
+                    f"ue=2^{feat[i][3]:.1f} lvl={feat[i][4]:.0f} fixed={dawn.MODE_NAMES[modes[i]]} "
+                    f"push={times[i][0]:.2f} pull={times[i][1]:.2f} bitmap={times[i][2]:.2f} ms")
     sched = {
         "subset_sources": int(len(sub)),
         "oracle_best_ms": regret["oracle_best"],

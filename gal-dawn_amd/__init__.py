@@ -121,7 +121,8 @@ def _sig(lib):
         "dawn_batch_mssp": (c.c_int, [c.POINTER(P), c.c_int32, _I32P, c.c_int32, P, _I64P, c.POINTER(Stats)]),
         "dawn_mssp_profile": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F64P, _I64P, _I32P]),
         "dawn_sched_set_mlp": (c.c_int, [_F32P, _F32P, _F32P]),
-        "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, c.c_int32, _I32P]),
+        "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, _I32P, c.c_int32,
+                                        _I32P]),
         "dawn_sched_trace": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _I32P, _F32P, _F32P, c.c_int32, _I32P]),
         "dawn_mlp_forward": (c.c_int, [_F32P, _F32P, c.c_int32, _F32P]),
         "dawn_mlp_loss_grad": (c.c_int, [_F32P, _F32P, _F32P, c.c_int32, _F32P, _F32P]),
@@ -434,12 +435,14 @@ class Graph:
         src = _i32(sources)
         feat = np.zeros((max_rows, SCHED_FEATURES), np.float32)
         times = np.zeros((max_rows, 3), np.float32)
+        fixed = np.zeros(max_rows, np.int32)
         rows = ctypes.c_int32(0)
         _check(lib().dawn_sched_record(self._h, _ptr(src, ctypes.c_int32), len(src), repeats,
                                        _ptr(feat, ctypes.c_float), _ptr(times, ctypes.c_float),
-                                       max_rows, ctypes.byref(rows)), "dawn_sched_record")
+                                       _ptr(fixed, ctypes.c_int32), max_rows, ctypes.byref(rows)),
+               "dawn_sched_record")
         r = rows.value
-        return feat[:r].copy(), times[:r].copy()
+        return feat[:r].copy(), times[:r].copy(), fixed[:r].copy()
 
     def sched_trace(self, sources, policy="fixed", max_rows: int = 4096):
         src = _i32(sources)

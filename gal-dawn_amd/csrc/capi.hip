@@ -203,13 +203,13 @@ int dawn_sched_set_mlp(const float* params, const float* feat_mean, const float*
 }
 
 int dawn_sched_record(const dawn_graph* gc, const int32_t* sources, int32_t count,
-                      int32_t repeats, float* features, float* times_ms, int32_t max_rows,
-                      int32_t* rows) {
+                      int32_t repeats, float* features, float* times_ms, int32_t* fixed_modes,
+                      int32_t max_rows, int32_t* rows) {
   dawn_graph* g = const_cast<dawn_graph*>(gc);
   if (!g || !features || !times_ms || !rows || max_rows < 0) return DAWN_ERR_INVALID_ARG;
   if (g->weighted) return DAWN_ERR_UNSUPPORTED;
-  return run_ms_record(g, sources, count, repeats, features, times_ms, nullptr, max_rows, rows,
-                       DAWN_POLICY_FIXED);
+  return run_ms_record(g, sources, count, repeats, features, times_ms, fixed_modes, max_rows,
+                       rows, DAWN_POLICY_FIXED);
 }
 
 int dawn_sched_trace(const dawn_graph* gc, const int32_t* sources, int32_t count, int32_t policy,

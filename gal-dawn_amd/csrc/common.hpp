@@ -99,6 +99,12 @@ struct dawn_graph {
   int32_t* hc_beg = nullptr;
   int32_t* hc_end = nullptr;
   int64_t num_hc = 0;
+  // Same for out-edge lists (dense top-down step).  Alias the hc_* arrays
+  // for undirected graphs.
+  int32_t* oc_v = nullptr;
+  int32_t* oc_beg = nullptr;
+  int32_t* oc_end = nullptr;
+  int64_t num_oc = 0;
   std::vector<int32_t> h_row_ptr; // host copy of offsets (degrees for drivers)
   hipStream_t stream = nullptr;
   std::mutex lock;                // one traversal per graph at a time

@@ -134,22 +134,38 @@ int dawn_graph_create(const dawn_csr* csr, dawn_graph** out) {
     if (csr->weight && (rc = up(&g->in_weight, iw.data(), (size_t)m))) return fail(rc);
   }
   {
-    // Heavy in-edge chunks for the pull step (see dawn_graph::hc_v).
-    const std::vector<int32_t>& irp = g->directed ? in_rp_host : g->h_row_ptr;
-    std::vector<int32_t> hv, hb, he;
-    for (int32_t v = 0; v < n; v++) {
-      const int32_t b = irp[v], e = irp[v + 1];
-      if (e - b < kHeavyDegree) continue;
-      for (int32_t k = b; k < e; k += kPullChunk) {
-        hv.push_back(v);
-        hb.push_back(k);
-        he.push_back(std::min(e, k + kPullChunk));
+    // Heavy in-edge chunks for the pull step (see dawn_graph::hc_v) and
+    // heavy out-edge chunks for the dense top-down step.
+    auto chunks = [&](const std::vector<int32_t>& rp, int32_t** dv, int32_t** db, int32_t** de,
+                      int64_t* count) -> int {
+      std::vector<int32_t> hv, hb, he;
+      for (int32_t v = 0; v < n; v++) {
+        const int32_t b = rp[v], e = rp[v + 1];
+        if (e - b < kHeavyDegree) continue;
+        for (int32_t k = b; k < e; k += kPullChunk) {
+          hv.push_back(v);
+          hb.push_back(k);
+          he.push_back(std::min(e, k + kPullChunk));
+        }
       }
+      *count = (int64_t)hv.size();
+      int r;
+      if ((r = up(dv, hv.data(), hv.size()))) return r;
+      if ((r = up(db, hb.data(), hb.size()))) return r;
+      return up(de, he.data(), he.size());
+    };
+    if ((rc = chunks(g->directed ? in_rp_host : g->h_row_ptr, &g->hc_v, &g->hc_beg, &g->hc_end,
+                     &g->num_hc)))
+      return fail(rc);
+    if (g->directed) {
+      if ((rc = chunks(g->h_row_ptr, &g->oc_v, &g->oc_beg, &g->oc_end, &g->num_oc)))
+        return fail(rc);
+    } else {
+      g->oc_v = g->hc_v;
+      g->oc_beg = g->hc_beg;
+      g->oc_end = g->hc_end;
+      g->num_oc = g->num_hc;
     }
-    g->num_hc = (int64_t)hv.size();
-    if ((rc = up(&g->hc_v, hv.data(), hv.size()))) return fail(rc);
-    if ((rc = up(&g->hc_beg, hb.data(), hb.size()))) return fail(rc);
-    if ((rc = up(&g->hc_end, he.data(), he.size()))) return fail(rc);
   }
   *out = g;
   return DAWN_OK;
@@ -165,6 +181,11 @@ int dawn_graph_destroy(dawn_graph* g) {
     if (g->in_row_ptr) (void)hipFree(g->in_row_ptr);
     if (g->in_col) (void)hipFree(g->in_col);
     if (g->in_weight) (void)hipFree(g->in_weight);
+  }
+  if (g->directed) {
+    if (g->oc_v) (void)hipFree(g->oc_v);
+    if (g->oc_beg) (void)hipFree(g->oc_beg);
+    if (g->oc_end) (void)hipFree(g->oc_end);
   }
   if (g->hc_v) (void)hipFree(g->hc_v);
   if (g->hc_beg) (void)hipFree(g->hc_beg);

@@ -74,6 +74,10 @@ struct MsArgs {
   const int32_t* hc_beg;
   const int32_t* hc_end;
   int64_t num_hc;
+  const int32_t* oc_v;    // heavy out-edge chunks (dense top-down)
+  const int32_t* oc_beg;
+  const int32_t* oc_end;
+  int64_t num_oc;
   LevelCounters* ring;
   SchedState* st;
   MsBatch* batch;
@@ -133,7 +137,9 @@ __device__ __forceinline__ int owner_of(const int* off, int e) {
 // Per-thread counters of a level, flushed once per wave.
 struct MsAcc {
   unsigned long long fv = 0, fe = 0, scout = 0, lanes = 0, uv = 0, ue = 0;
+  uint64_t alive = 0;
   __device__ void flush(LevelCounters* c) {
+    const uint64_t al = wave_or_u64(alive);
     unsigned long long a0 = wave_sum_u64(fv), a1 = wave_sum_u64(fe), a2 = wave_sum_u64(scout);
     unsigned long long a3 = wave_sum_u64(lanes), a4 = wave_sum_u64(uv), a5 = wave_sum_u64(ue);
     if (lane_id() == 0) {
@@ -147,6 +153,7 @@ struct MsAcc {
         atomicAdd(&c->uv, a4);
         atomicAdd(&c->ue, a5);
       }
+      if (al) atomicOr((unsigned long long*)&c->alive, (unsigned long long)al);
     }
   }
 };
@@ -157,7 +164,7 @@ struct MsAcc {
 __device__ __forceinline__ void ms_commit(const MsArgs& a, bool valid, int32_t v,
                                           uint64_t sv, uint64_t nw, uint64_t* slot_out,
                                           int32_t* q_out, int32_t* qb_out, unsigned* qlen,
-                                          MsAcc& acc) {
+                                          uint64_t alive_prev, MsAcc& acc) {
   if (valid) {
     slot_out[v] = nw;
     if (nw) a.seen[v] = sv | nw;
@@ -165,6 +172,13 @@ __device__ __forceinline__ void ms_commit(const MsArgs& a, bool valid, int32_t v
   const bool isf = valid && nw != 0;
   // A frontier vertex takes one queue entry per kPushChunk out-edges, so a
   // push step's work per entry is bounded whatever the degree.
+  if (__ballot(isf) == 0) {  // wave-uniform: nothing to append
+    if (valid && (~(sv | nw) & alive_prev) != 0) {
+      acc.uv += 1;
+      acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+    }
+    return;
+  }
   const int d = isf ? out_deg(a, v) : 0;
   const int k = (d + kPushChunk - 1) / kPushChunk;
   const int pos = wave_append_n(qlen, k);
@@ -178,8 +192,9 @@ __device__ __forceinline__ void ms_commit(const MsArgs& a, bool valid, int32_t v
     acc.fe += (unsigned)d;
     acc.scout += (unsigned)(d > 0 ? d : 1);
     acc.lanes += (unsigned)__popcll(nw);
+    acc.alive |= nw;
   }
-  if (valid && (~(sv | nw)) != 0) {
+  if (valid && (~(sv | nw) & alive_prev) != 0) {
     acc.uv += 1;
     acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
   }
@@ -225,6 +240,7 @@ __global__ void ms_seed_kernel(MsArgs a) {
     a.hist[v] |= bit;
     a.seen[v] |= bit;
     c.flanes += 1;
+    c.alive |= bit;
   }
   // Vertices still missing a lane after level 0, and their in-degree sum.
   c.uv = (unsigned long long)a.n;
@@ -294,57 +310,61 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       __syncthreads();
     }
   } else if (st.mode == DAWN_MODE_BITMAP) {
+    // Dense top-down: light frontier vertices one lane each, heavy ones in
+    // out-edge chunks of kPullChunk edges, one wave per chunk.
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
       const int32_t u = (int32_t)(base + lane);
       const uint64_t f = u < a.n ? prev[u] : 0ull;
-      const bool act = f != 0;
-      if (__ballot(act) == 0) continue;
-      int beg = 0, deg = 0;
-      if (act) {
-        beg = a.row_ptr[u];
-        deg = a.row_ptr[u + 1] - beg;
+      if (!f) continue;
+      const int beg = a.row_ptr[u];
+      const int deg = a.row_ptr[u + 1] - beg;
+      if (deg >= kHeavyDegree) continue;
+      for (int j = 0; j < deg; j++) {
+        const int32_t v = a.col[beg + j];
+        const uint64_t nv = f & ~a.seen[v];
+        if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
       }
-      const bool light = act && deg < kHeavyDegree;
-      if (light) {
-        for (int j = 0; j < deg; j++) {
-          const int32_t v = a.col[beg + j];
-          const uint64_t nv = f & ~a.seen[v];
-          if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
-        }
-      }
-      uint64_t hm = __ballot(act && deg >= kHeavyDegree);
-      while (hm) {
-        const int l = __ffsll((unsigned long long)hm) - 1;
-        hm &= hm - 1;
-        const int hb = __shfl(beg, l), he = hb + __shfl(deg, l);
-        const uint64_t hf = ((uint64_t)__shfl((unsigned)(f >> 32), l) << 32) |
-                            (unsigned)__shfl((unsigned)(f & 0xffffffffu), l);
-        for (int k = hb + lane; k < he; k += kWave) {
-          const int32_t v = a.col[k];
-          const uint64_t nv = hf & ~a.seen[v];
-          if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
-        }
+    }
+    const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t c = wave; c < a.num_oc; c += nwaves) {
+      const int32_t u = a.oc_v[c];
+      const uint64_t f = prev[u];
+      if (!f) continue;
+      const int e = a.oc_end[c];
+      for (int k = a.oc_beg[c] + lane; k < e; k += kWave) {
+        const int32_t v = a.col[k];
+        const uint64_t nv = f & ~a.seen[v];
+        if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
       }
     }
   } else {  // DAWN_MODE_PULL
-    // Light vertices (in-degree < kHeavyDegree): one lane each, early exit
-    // once every missing lane is found; the lane owns next[v].
+    // Only lanes alive in the previous level can arrive now: a vertex looks
+    // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
+    const uint64_t alive = a.ring[(lv - 1) % kRing].alive;
+    // Light vertices (in-degree < kHeavyDegree): one lane each, four gathers
+    // in flight between early-exit checks; the lane owns next[v].
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
       const int32_t v = (int32_t)(base + lane);
-      const uint64_t miss = v < a.n ? ~a.seen[v] : 0ull;
+      const uint64_t miss = v < a.n ? (~a.seen[v] & alive) : 0ull;
       if (!miss) continue;
       const int beg = a.in_row_ptr[v];
       const int deg = a.in_row_ptr[v + 1] - beg;
       if (deg >= kHeavyDegree || deg == 0) continue;
+      const int32_t* nb = a.in_col + beg;
       uint64_t acc_w = 0;
-      for (int j = 0; j < deg; j++) {
-        acc_w |= prev[a.in_col[beg + j]];
+      int j = 0;
+      for (; j + 4 <= deg; j += 4) {
+        const int32_t u0 = nb[j], u1 = nb[j + 1], u2 = nb[j + 2], u3 = nb[j + 3];
+        acc_w |= prev[u0] | prev[u1] | prev[u2] | prev[u3];
         if ((acc_w & miss) == miss) break;
       }
+      if ((acc_w & miss) != miss)
+        for (; j < deg; j++) acc_w |= prev[nb[j]];
       if (acc_w & miss) a.next[v] = acc_w & miss;
     }
     // Heavy vertices: their in-edge lists are cut into chunks of at most
@@ -353,7 +373,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t c = wave; c < a.num_hc; c += nwaves) {
       const int32_t v = a.hc_v[c];
-      const uint64_t miss = ~a.seen[v];
+      const uint64_t miss = ~a.seen[v] & alive;
       if (!miss) continue;
       if ((a.next[v] & miss) == miss) continue;  // other chunks already found them all
       const int he = a.hc_end[c];
@@ -374,6 +394,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   int32_t* q_out = a.q + (int64_t)(lv & 1) * a.qcap;
   int32_t* qb_out = a.qb + (int64_t)(lv & 1) * a.qcap;
   LevelCounters* c_out = a.ring + (lv % kRing);
+  const uint64_t alive_prev = a.ring[(lv - 1) % kRing].alive;
   MsAcc acc;
   const int64_t wstride = (int64_t)gridDim.x * kBlock;
   for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.npad;
@@ -387,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       if (nx) a.next[v] = 0ull;
       nw = nx & ~sv;
     }
-    ms_commit(a, valid, v, sv, nw, cur, q_out, qb_out, &c_out->qlen, acc);
+    ms_commit(a, valid, v, sv, nw, cur, q_out, qb_out, &c_out->qlen, alive_prev, acc);
   }
   acc.flush(c_out);
 }
@@ -535,6 +556,10 @@ MsArgs make_args(dawn_graph* g) {
   a.hc_beg = g->hc_beg;
   a.hc_end = g->hc_end;
   a.num_hc = g->num_hc;
+  a.oc_v = g->oc_v;
+  a.oc_beg = g->oc_beg;
+  a.oc_end = g->oc_end;
+  a.num_oc = g->num_oc;
   a.ring = ws->ring.p;
   a.st = ws->st.p;
   a.batch = ws->batch.p;

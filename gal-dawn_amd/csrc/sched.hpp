@@ -17,8 +17,10 @@ struct LevelCounters {
   unsigned long long fe;      // sum of out-degree over them
   unsigned long long scout;   // sum of max(out-degree, 1) (GAPBS scout_count)
   unsigned long long flanes;  // sum of popcount of frontier words (batched)
-  unsigned long long uv;      // vertices still missing a lane after this level
+  unsigned long long uv;      // vertices still missing a live lane after this level
   unsigned long long ue;      // sum of in-degree over those (pull's edges to check)
+  unsigned long long alive;   // batched: OR of the level's frontier words (lanes
+                              // that can still reach new vertices)
   unsigned int qlen;          // queue entries written for this level
   unsigned int pad;
 };

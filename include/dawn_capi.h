@@ -217,12 +217,15 @@ int dawn_sched_set_mlp(const float* params, const float* feat_mean,
                        const float* feat_inv_std);
 
 /* Record per-step mode timings: for each batch step every mode is run from
- * the same state and timed with HIP events; then the engine advances.
- * features: rows*8 floats, times_ms: rows*3 floats.  *rows receives the
- * number of rows written (<= max_rows).  Unweighted graphs only. */
+ * the same state and timed with HIP events (minimum of `repeats`); then the
+ * engine advances.  features: rows*8 floats, times_ms: rows*3 floats,
+ * fixed_modes (nullable): rows ints, the mode the fixed alpha/beta rule
+ * chose at that step.  *rows receives the number of rows written
+ * (<= max_rows).  Unweighted graphs only. */
 int dawn_sched_record(const dawn_graph* g, const int32_t* sources,
                       int32_t count, int32_t repeats, float* features,
-                      float* times_ms, int32_t max_rows, int32_t* rows);
+                      float* times_ms, int32_t* fixed_modes, int32_t max_rows,
+                      int32_t* rows);
 
 /* Per-step trace of a policy run: modes[i] and features (rows*8). */
 int dawn_sched_trace(const dawn_graph* g, const int32_t* sources,
