@@ -95,7 +95,8 @@ def main():
         meta = json.loads(str(d["meta"]))
     else:
         X, T, FX, meta = record(args.quick, args.repeats, args.batches)
-        np.savez(os.path.join(HERE, "sched_data.npz"), features=X, times_ms=T, fixed_modes=FX,
+        np.savez(os.path.join(os.path.dirname(os.path.abspath(args.out)), "sched_data.npz"),
+                 features=X, times_ms=T, fixed_modes=FX,
                  meta=json.dumps(meta))
     Y = np.log2(np.maximum(T, 1e-4) * 1e3).astype(np.float32)  # log2 microseconds
     mean = X.mean(axis=0)
