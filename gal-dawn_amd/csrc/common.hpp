@@ -154,6 +154,35 @@ __device__ __forceinline__ int wave_append_n(unsigned int* counter, int k) {
   return (int)base + inc - k;
 }
 
+// Block-aggregated append of k >= 0 slots per thread: one atomic per block.
+// Returns the thread's first slot.  Every thread of the kBlock-thread block
+// must call it (it synchronises the block).  s_wtot: kBlock/kWave ints of
+// LDS, s_base: one unsigned of LDS.
+__device__ __forceinline__ int block_append_n(unsigned int* counter, int k, int* s_wtot,
+                                              unsigned int* s_base) {
+  const int l = lane_id();
+  const int w = threadIdx.x / kWave;
+  int inc = k;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(inc, o);
+    if (l >= o) inc += y;
+  }
+  if (l == kWave - 1) s_wtot[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int i = 0; i < kBlock / kWave; i++) t += s_wtot[i];
+    *s_base = t ? atomicAdd(counter, (unsigned int)t) : 0u;
+  }
+  __syncthreads();
+  int off = 0;
+  for (int i = 0; i < w; i++) off += s_wtot[i];
+  const int r = (int)*s_base + off + inc - k;
+  __syncthreads();  // s_wtot / s_base may be reused after this
+  return r;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
   for (int o = 32; o >= 1; o >>= 1) {
     unsigned lo = __shfl_xor((unsigned)(x & 0xffffffffu), o);

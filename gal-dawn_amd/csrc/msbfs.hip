@@ -134,71 +134,41 @@ __device__ __forceinline__ int owner_of(const int* off, int e) {
   return lo;
 }
 
-// Per-thread counters of a level, flushed once per wave.
+// Per-thread counters of a level, reduced over the workgroup and flushed
+// with one set of atomics per workgroup (same-address atomics from every
+// wave serialise in L2).  flush() must be called by every thread.
 struct MsAcc {
   unsigned long long fv = 0, fe = 0, scout = 0, lanes = 0, uv = 0, ue = 0;
   uint64_t alive = 0;
   __device__ void flush(LevelCounters* c) {
-    const uint64_t al = wave_or_u64(alive);
-    unsigned long long a0 = wave_sum_u64(fv), a1 = wave_sum_u64(fe), a2 = wave_sum_u64(scout);
-    unsigned long long a3 = wave_sum_u64(lanes), a4 = wave_sum_u64(uv), a5 = wave_sum_u64(ue);
-    if (lane_id() == 0) {
-      if (a0) {
-        atomicAdd(&c->fv, a0);
-        atomicAdd(&c->fe, a1);
-        atomicAdd(&c->scout, a2);
-        atomicAdd(&c->flanes, a3);
+    __shared__ unsigned long long s_v[kBlock / kWave][7];
+    const int w = threadIdx.x / kWave;
+    const unsigned long long r[7] = {wave_sum_u64(fv),    wave_sum_u64(fe), wave_sum_u64(scout),
+                                     wave_sum_u64(lanes), wave_sum_u64(uv), wave_sum_u64(ue),
+                                     wave_or_u64(alive)};
+    if (lane_id() == 0)
+      for (int i = 0; i < 7; i++) s_v[w][i] = r[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t[7] = {0, 0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < kBlock / kWave; j++) {
+        for (int i = 0; i < 6; i++) t[i] += s_v[j][i];
+        t[6] |= s_v[j][6];
       }
-      if (a4) {
-        atomicAdd(&c->uv, a4);
-        atomicAdd(&c->ue, a5);
+      if (t[0]) {
+        atomicAdd(&c->fv, t[0]);
+        atomicAdd(&c->fe, t[1]);
+        atomicAdd(&c->scout, t[2]);
+        atomicAdd(&c->flanes, t[3]);
       }
-      if (al) atomicOr((unsigned long long*)&c->alive, (unsigned long long)al);
+      if (t[4]) {
+        atomicAdd(&c->uv, t[4]);
+        atomicAdd(&c->ue, t[5]);
+      }
+      if (t[6]) atomicOr((unsigned long long*)&c->alive, t[6]);
     }
   }
 };
-
-// Commits the new lanes nw of vertex v (v < n) for level lv: level slot,
-// seen, queue, counters.  Called by every lane of a wave (queue append is
-// wave-aggregated); lanes with v >= n pass valid = false.
-__device__ __forceinline__ void ms_commit(const MsArgs& a, bool valid, int32_t v,
-                                          uint64_t sv, uint64_t nw, uint64_t* slot_out,
-                                          int32_t* q_out, int32_t* qb_out, unsigned* qlen,
-                                          uint64_t alive_prev, MsAcc& acc) {
-  if (valid) {
-    slot_out[v] = nw;
-    if (nw) a.seen[v] = sv | nw;
-  }
-  const bool isf = valid && nw != 0;
-  // A frontier vertex takes one queue entry per kPushChunk out-edges, so a
-  // push step's work per entry is bounded whatever the degree.
-  if (__ballot(isf) == 0) {  // wave-uniform: nothing to append
-    if (valid && (~(sv | nw) & alive_prev) != 0) {
-      acc.uv += 1;
-      acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
-    }
-    return;
-  }
-  const int d = isf ? out_deg(a, v) : 0;
-  const int k = (d + kPushChunk - 1) / kPushChunk;
-  const int pos = wave_append_n(qlen, k);
-  if (isf) {
-    const int32_t rb = a.row_ptr[v];
-    for (int i = 0; i < k; i++) {
-      q_out[pos + i] = v;
-      qb_out[pos + i] = rb + i * kPushChunk;
-    }
-    acc.fv += 1;
-    acc.fe += (unsigned)d;
-    acc.scout += (unsigned)(d > 0 ? d : 1);
-    acc.lanes += (unsigned)__popcll(nw);
-    acc.alive |= nw;
-  }
-  if (valid && (~(sv | nw) & alive_prev) != 0) {
-    acc.uv += 1;
-    acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
-  }
-}
 
 // ---------------------------------------------------------------- kernels ---
 
@@ -387,6 +357,8 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
 
 // Folds next[] into the level slot, seen, the queue and the counters.
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
+  __shared__ int s_wtot[kBlock / kWave];
+  __shared__ unsigned s_base;
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
@@ -396,10 +368,9 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   LevelCounters* c_out = a.ring + (lv % kRing);
   const uint64_t alive_prev = a.ring[(lv - 1) % kRing].alive;
   MsAcc acc;
-  const int64_t wstride = (int64_t)gridDim.x * kBlock;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.npad;
-       base += wstride) {
-    const int32_t v = (int32_t)(base + lane_id());
+  // Block-uniform loop: every thread of a workgroup runs the same trips.
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.npad; b0 += (int64_t)gridDim.x * kBlock) {
+    const int32_t v = (int32_t)(b0 + threadIdx.x);
     const bool valid = v < a.n;
     uint64_t sv = ~0ull, nw = 0;
     if (valid) {
@@ -407,8 +378,33 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       sv = a.seen[v];
       if (nx) a.next[v] = 0ull;
       nw = nx & ~sv;
+      cur[v] = nw;
+      if (nw) a.seen[v] = sv | nw;
+      if ((~(sv | nw) & alive_prev) != 0) {  // still misses a live lane
+        acc.uv += 1;
+        acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+      }
     }
-    ms_commit(a, valid, v, sv, nw, cur, q_out, qb_out, &c_out->qlen, alive_prev, acc);
+    const bool isf = valid && nw != 0;
+    if (__syncthreads_or(isf)) {
+      // One queue entry per kPushChunk out-edges of each frontier vertex,
+      // so a push step's work per entry is bounded whatever the degree.
+      const int d = isf ? out_deg(a, v) : 0;
+      const int k = (d + kPushChunk - 1) / kPushChunk;
+      const int pos = block_append_n(&c_out->qlen, k, s_wtot, &s_base);
+      if (isf) {
+        const int32_t rb = a.row_ptr[v];
+        for (int i = 0; i < k; i++) {
+          q_out[pos + i] = v;
+          qb_out[pos + i] = rb + i * kPushChunk;
+        }
+        acc.fv += 1;
+        acc.fe += (unsigned)d;
+        acc.scout += (unsigned)(d > 0 ? d : 1);
+        acc.lanes += (unsigned)__popcll(nw);
+        acc.alive |= nw;
+      }
+    }
   }
   acc.flush(c_out);
 }
@@ -582,7 +578,8 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
   return p;
 }
 
-unsigned ms_grid(const dawn_graph* g) { return grid_for(std::max<int64_t>(g->n, 1), kBlock, 8192); }
+// Dense step kernels: at most 8 workgroups per CU (2048), grid-stride beyond.
+unsigned ms_grid(const dawn_graph* g) { return grid_for(std::max<int64_t>(g->n, 1), kBlock, 2048); }
 
 // Stages one batch's sources (taken mod n) into the pinned batch block.
 void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
