@@ -1,0 +1,414 @@
+// Many independent graphs of uneven size in one persistent launch.
+//
+// Replaces, for a batch of graphs, the reference's host loop of per-source
+// launches (DAWN::MSSP_GPU::run, src/algorithm/gpu/mssp.cu:26-43 and
+// run_Weighted :53-102): here every (graph, source block) is a task in a
+// device work queue, and each workgroup pops tasks with one atomic and runs
+// the whole traversal inside the workgroup (barriers are workgroup-local, so
+// no grid-wide synchronisation exists and every workgroup exits once the
+// task counter passes the last task).
+//
+// Unweighted task: all sources of one graph as lanes of a uint64 word
+// (<= 64 lanes), sparse level-synchronous BOVM:
+//   Q_l = [(v, lanes that reached v at level l)]
+//   expand: for (u, f) in Q_l, w in N(u): nv = f & ~seen[w]; if nv,
+//           atomicOr(next[w], nv); the first writer appends w to `touched`
+//   commit: for w in touched: nw = next[w] & ~seen[w]; next[w] = 0;
+//           seen[w] |= nw; dist[b][w] = l+1 for b in nw; Q_{l+1} += (w, nw)
+// Weighted task: one source, int32 min-plus (GOVM semantics,
+// src/algorithm/cpu/sssp.cpp:141-199): Q_l = vertices whose distance dropped
+// in the previous step; atomicMin relaxations in place in the output row.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace dawn {
+
+struct BatchGraph {
+  int32_t n;
+  int32_t weighted;
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* w;
+  int64_t dist_off;  // first output row of this graph
+};
+
+struct BatchTask {
+  int32_t graph;
+  int32_t lane0;   // first source index within the graph's block
+  int32_t lanes;   // sources in this task (1 for weighted)
+  int32_t pad;
+};
+
+struct BatchArgs {
+  const BatchGraph* graphs;
+  const BatchTask* tasks;
+  int32_t num_tasks;
+  int32_t spg;               // sources per graph
+  const int32_t* sources;    // num_graphs * spg (already reduced mod n)
+  int32_t* dist;
+  unsigned* task_counter;
+  unsigned long long* levels_total;
+  // Per-workgroup scratch, sized for the largest graph (max_n).
+  int64_t max_n;
+  uint64_t* seen;      // [grid][max_n]
+  uint64_t* next;      // [grid][max_n]
+  int32_t* qv;         // [grid][2][max_n]
+  uint64_t* qw;        // [grid][2][max_n]
+  int32_t* touched;    // [grid][max_n]
+};
+
+namespace {
+
+constexpr int kBatchBlock = kBlock;
+
+__device__ __forceinline__ int owner(const int* off, int e) {
+  int lo = 0, hi = kBatchBlock - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Appends through an LDS counter (one workgroup owns the list).
+__device__ __forceinline__ int lds_append(unsigned* s_count, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return -1;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned base = 0;
+  if (lane_id() == leader) base = atomicAdd(s_count, (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  return pred ? (int)(base + (unsigned)__popcll(m & lanes_below_mask())) : -1;
+}
+
+__device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const BatchTask& T,
+                               uint64_t* seen, uint64_t* next, int32_t* qv, uint64_t* qw,
+                               int32_t* touched, int* s_off, int* s_beg, uint64_t* s_f,
+                               int* s_scan, unsigned* s_cnt) {
+  const int32_t n = G.n;
+  const int64_t mn = a.max_n;
+  int32_t* drow = a.dist + G.dist_off;  // lane b row: drow + b * n
+  const uint64_t lanemask = T.lanes >= 64 ? ~0ull : ((1ull << T.lanes) - 1);
+  for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
+    seen[v] = ~lanemask;
+    next[v] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_cnt[0] = 0;  // |Q_0|
+    for (int i = 0; i < T.lanes; i++) {
+      const int32_t s = a.sources[(int64_t)T.graph * a.spg + T.lane0 + i];
+      const uint64_t bit = 1ull << i;
+      if ((seen[s] & lanemask) == 0) {
+        qv[s_cnt[0]] = s;
+        qw[s_cnt[0]] = 0;
+        s_cnt[0]++;
+      }
+      seen[s] |= bit;
+      drow[(int64_t)i * n + s] = 0;
+    }
+    // Words of the level-0 entries.
+    for (unsigned j = 0; j < s_cnt[0]; j++) qw[j] = seen[qv[j]] & lanemask;
+  }
+  __syncthreads();
+  unsigned qn = s_cnt[0];
+  int par = 0;
+  int level = 0;
+  while (qn > 0) {
+    const int32_t* qv_in = qv + (int64_t)par * mn;
+    const uint64_t* qw_in = qw + (int64_t)par * mn;
+    int32_t* qv_out = qv + (int64_t)(par ^ 1) * mn;
+    uint64_t* qw_out = qw + (int64_t)(par ^ 1) * mn;
+    if (threadIdx.x == 0) s_cnt[1] = 0;  // |touched|
+    __syncthreads();
+    // Expand Q_level with workgroup edge balancing (LDS degree scan).
+    for (unsigned c0 = 0; c0 < qn; c0 += kBatchBlock) {
+      const unsigned i = c0 + threadIdx.x;
+      int beg = 0, deg = 0;
+      uint64_t f = 0;
+      if (i < qn) {
+        const int32_t u = qv_in[i];
+        beg = G.row_ptr[u];
+        deg = G.row_ptr[u + 1] - beg;
+        f = qw_in[i];
+      }
+      int total;
+      const int off = block_exclusive_scan(deg, s_scan, &total);
+      s_off[threadIdx.x] = off;
+      s_beg[threadIdx.x] = beg;
+      s_f[threadIdx.x] = f;
+      __syncthreads();
+      for (int e0 = 0; e0 < total; e0 += kBatchBlock) {
+        const int e = e0 + threadIdx.x;
+        bool first = false;
+        int32_t w = 0;
+        if (e < total) {
+          const int j = owner(s_off, e);
+          w = G.col[s_beg[j] + (e - s_off[j])];
+          const uint64_t nv = s_f[j] & ~seen[w];
+          if (nv) first = atomicOr((unsigned long long*)&next[w], (unsigned long long)nv) == 0ull;
+        }
+        const int slot = lds_append(&s_cnt[1], first);
+        if (first) touched[slot] = w;
+      }
+      __syncthreads();
+    }
+    // Commit the touched vertices into Q_{level+1} and the distance rows.
+    if (threadIdx.x == 0) s_cnt[2] = 0;
+    __syncthreads();
+    const unsigned tn = s_cnt[1];
+    const int32_t lv = level + 1;
+    for (unsigned t0 = 0; t0 < tn; t0 += kBatchBlock) {
+      const unsigned t = t0 + threadIdx.x;
+      bool isf = false;
+      int32_t w = 0;
+      uint64_t nw = 0;
+      if (t < tn) {
+        w = touched[t];
+        const uint64_t nx = next[w];
+        next[w] = 0;
+        nw = nx & ~seen[w];
+        if (nw) {
+          seen[w] |= nw;
+          isf = true;
+          uint64_t bits = nw;
+          while (bits) {
+            const int b = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            drow[(int64_t)b * n + w] = lv;
+          }
+        }
+      }
+      const int slot = lds_append(&s_cnt[2], isf);
+      if (isf) {
+        qv_out[slot] = w;
+        qw_out[slot] = nw;
+      }
+    }
+    __syncthreads();
+    qn = s_cnt[2];
+    par ^= 1;
+    level = lv;
+    __syncthreads();
+  }
+  // Lanes that never arrived: INF.
+  for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
+    uint64_t miss = ~seen[v] & lanemask;
+    while (miss) {
+      const int b = __ffsll((unsigned long long)miss) - 1;
+      miss &= miss - 1;
+      drow[(int64_t)b * n + v] = kInf;
+    }
+  }
+  if (threadIdx.x == 0) atomicAdd(a.levels_total, (unsigned long long)level);
+}
+
+// One weighted source.  The reference's initialisation
+// (src/algorithm/cpu/sssp.cpp:40-46): listed out-edges of the source write
+// their weight in row order, the distinct neighbours form the first
+// frontier, distance[source] = 0, the source is never relaxed.
+__device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const BatchTask& T,
+                             uint32_t* mark, int32_t* qv, int32_t* touched, int* s_off,
+                             int* s_beg, int* s_du, int* s_scan, unsigned* s_cnt) {
+  const int32_t n = G.n;
+  const int64_t mn = a.max_n;
+  const int32_t src = a.sources[(int64_t)T.graph * a.spg + T.lane0];
+  int32_t* dist = a.dist + G.dist_off + (int64_t)T.lane0 * n;
+  for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
+    dist[v] = kInf;
+    mark[v] = 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int32_t b = G.row_ptr[src], e = G.row_ptr[src + 1];
+    for (int32_t k = b; k < e; k++) dist[G.col[k]] = G.w[k];
+    dist[src] = 0;
+    unsigned c = 0;
+    for (int32_t k = b; k < e; k++) {
+      const int32_t v = G.col[k];
+      if (v == src || mark[v] == 1u) continue;
+      mark[v] = 1u;
+      qv[c++] = v;
+    }
+    s_cnt[0] = c;
+  }
+  __syncthreads();
+  unsigned qn = s_cnt[0];
+  int par = 0;
+  uint32_t step = 1;
+  while (qn > 0) {
+    const int32_t* q_in = qv + (int64_t)par * mn;
+    int32_t* q_out = qv + (int64_t)(par ^ 1) * mn;
+    step++;
+    if (threadIdx.x == 0) s_cnt[1] = 0;
+    __syncthreads();
+    for (unsigned c0 = 0; c0 < qn; c0 += kBatchBlock) {
+      const unsigned i = c0 + threadIdx.x;
+      int beg = 0, deg = 0, du = 0;
+      if (i < qn) {
+        const int32_t u = q_in[i];
+        beg = G.row_ptr[u];
+        deg = G.row_ptr[u + 1] - beg;
+        du = dist[u];
+      }
+      int total;
+      const int off = block_exclusive_scan(deg, s_scan, &total);
+      s_off[threadIdx.x] = off;
+      s_beg[threadIdx.x] = beg;
+      s_du[threadIdx.x] = du;
+      __syncthreads();
+      for (int e0 = 0; e0 < total; e0 += kBatchBlock) {
+        const int e = e0 + threadIdx.x;
+        bool add = false;
+        int32_t v = 0;
+        if (e < total) {
+          const int j = owner(s_off, e);
+          const int k = s_beg[j] + (e - s_off[j]);
+          v = G.col[k];
+          const long long nd = (long long)s_du[j] + G.w[k];
+          if (v != src && nd < kInf && (int32_t)nd < dist[v]) {
+            const int32_t old = atomicMin(&dist[v], (int32_t)nd);
+            if ((int32_t)nd < old) add = atomicExch(&mark[v], step) != step;
+          }
+        }
+        const int slot = lds_append(&s_cnt[1], add);
+        if (add) q_out[slot] = v;
+      }
+      __syncthreads();
+    }
+    qn = s_cnt[1];
+    par ^= 1;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(a.levels_total, (unsigned long long)(step - 1));
+}
+
+__global__ void __launch_bounds__(kBatchBlock) batch_kernel(BatchArgs a) {
+  __shared__ int s_off[kBatchBlock];
+  __shared__ int s_beg[kBatchBlock];
+  __shared__ uint64_t s_f[kBatchBlock];
+  __shared__ int s_scan[kBatchBlock / kWave];
+  __shared__ unsigned s_cnt[4];
+  __shared__ int s_task;
+  const int64_t mn = a.max_n;
+  uint64_t* seen = a.seen + (int64_t)blockIdx.x * mn;
+  uint64_t* next = a.next + (int64_t)blockIdx.x * mn;
+  int32_t* qv = a.qv + (int64_t)blockIdx.x * 2 * mn;
+  uint64_t* qw = a.qw + (int64_t)blockIdx.x * 2 * mn;
+  int32_t* touched = a.touched + (int64_t)blockIdx.x * mn;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = (int)atomicAdd(a.task_counter, 1u);
+    __syncthreads();
+    const int t = s_task;
+    __syncthreads();
+    if (t >= a.num_tasks) break;  // every workgroup leaves once the queue is empty
+    const BatchTask T = a.tasks[t];
+    const BatchGraph G = a.graphs[T.graph];
+    if (G.weighted)
+      run_weighted(a, G, T, (uint32_t*)next, qv, touched, s_off, s_beg, (int*)s_f, s_scan, s_cnt);
+    else
+      run_unweighted(a, G, T, seen, next, qv, qw, touched, s_off, s_beg, s_f, s_scan, s_cnt);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+}  // namespace dawn
+
+using namespace dawn;
+
+extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
+                               const int32_t* sources, int32_t spg, int32_t* dist,
+                               const int64_t* dist_offsets, dawn_stats* stats) {
+  if (!graphs || num_graphs <= 0 || !sources || spg <= 0 || spg > 64 || !dist_offsets)
+    return DAWN_ERR_INVALID_ARG;
+  const int device = graphs[0]->device;
+  int64_t max_n = 1, total = 0;
+  std::vector<BatchGraph> hg(num_graphs);
+  std::vector<BatchTask> ht;
+  std::vector<int32_t> hs((size_t)num_graphs * spg);
+  for (int32_t i = 0; i < num_graphs; i++) {
+    dawn_graph* g = graphs[i];
+    if (!g || g->device != device || g->n <= 0) return DAWN_ERR_INVALID_ARG;
+    max_n = std::max<int64_t>(max_n, g->n);
+    hg[i] = BatchGraph{g->n, g->weighted, g->row_ptr, g->col, g->weight, dist_offsets[i]};
+    total = std::max<int64_t>(total, dist_offsets[i] + (int64_t)spg * g->n);
+    for (int32_t j = 0; j < spg; j++) {
+      int64_t s = sources[(int64_t)i * spg + j] % (int64_t)g->n;
+      if (s < 0) s += g->n;
+      hs[(size_t)i * spg + j] = (int32_t)s;
+      if (stats && g->h_row_ptr[s] == g->h_row_ptr[s + 1]) stats->skipped_sources += 1;
+    }
+    if (g->weighted)
+      for (int32_t j = 0; j < spg; j++) ht.push_back(BatchTask{i, j, 1, 0});
+    else
+      ht.push_back(BatchTask{i, 0, spg, 0});
+  }
+  // Largest graphs first: the long tasks start early.
+  std::stable_sort(ht.begin(), ht.end(), [&](const BatchTask& x, const BatchTask& y) {
+    return hg[x.graph].n > hg[y.graph].n;
+  });
+  DAWN_HIP(hipSetDevice(device));
+  int cus = 0;
+  DAWN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int grid = std::max(1, std::min<int>((int)ht.size(), cus * 2));
+
+  DevBuf<BatchGraph> dg;
+  DevBuf<BatchTask> dt;
+  DevBuf<int32_t> ds, dq, dtouch, dout;
+  DevBuf<uint64_t> dseen, dnext, dqw;
+  DevBuf<unsigned> dcnt;
+  DevBuf<unsigned long long> dlev;
+  DAWN_TRY(dg.alloc(num_graphs));
+  DAWN_TRY(dt.alloc(ht.size()));
+  DAWN_TRY(ds.alloc(hs.size()));
+  DAWN_TRY(dseen.alloc((size_t)grid * max_n));
+  DAWN_TRY(dnext.alloc((size_t)grid * max_n));
+  DAWN_TRY(dq.alloc((size_t)grid * 2 * max_n));
+  DAWN_TRY(dqw.alloc((size_t)grid * 2 * max_n));
+  DAWN_TRY(dtouch.alloc((size_t)grid * max_n));
+  DAWN_TRY(dcnt.alloc(1));
+  DAWN_TRY(dlev.alloc(1));
+  const bool dev_out = is_device_pointer(dist);
+  int32_t* out = dist;
+  if (!dev_out) {
+    DAWN_TRY(dout.alloc((size_t)total));
+    out = dout.p;
+  }
+  hipStream_t s = graphs[0]->stream;
+  DAWN_HIP(hipMemcpyAsync(dg.p, hg.data(), sizeof(BatchGraph) * hg.size(), hipMemcpyHostToDevice, s));
+  DAWN_HIP(hipMemcpyAsync(dt.p, ht.data(), sizeof(BatchTask) * ht.size(), hipMemcpyHostToDevice, s));
+  DAWN_HIP(hipMemcpyAsync(ds.p, hs.data(), sizeof(int32_t) * hs.size(), hipMemcpyHostToDevice, s));
+  DAWN_HIP(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned), s));
+  DAWN_HIP(hipMemsetAsync(dlev.p, 0, sizeof(unsigned long long), s));
+  BatchArgs a{dg.p,   dt.p,   (int32_t)ht.size(), spg,      ds.p,    out,
+              dcnt.p, dlev.p, max_n,              dseen.p, dnext.p, dq.p,
+              dqw.p,  dtouch.p};
+  hipEvent_t e0, e1;
+  DAWN_HIP(hipEventCreate(&e0));
+  DAWN_HIP(hipEventCreate(&e1));
+  DAWN_HIP(hipEventRecord(e0, s));
+  batch_kernel<<<grid, kBatchBlock, 0, s>>>(a);
+  DAWN_HIP(hipGetLastError());
+  DAWN_HIP(hipEventRecord(e1, s));
+  DAWN_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  DAWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  unsigned long long levels = 0;
+  DAWN_HIP(hipMemcpy(&levels, dlev.p, sizeof levels, hipMemcpyDeviceToHost));
+  if (!dev_out)
+    DAWN_HIP(hipMemcpy(dist, out, sizeof(int32_t) * (size_t)total, hipMemcpyDeviceToHost));
+  if (stats) {
+    stats->device_ms += ms;
+    stats->steps += (int64_t)levels;
+    stats->batches += (int64_t)ht.size();
+  }
+  return DAWN_OK;
+}

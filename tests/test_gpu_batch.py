@@ -1,0 +1,51 @@
+"""Uneven multi-graph batch (one persistent launch) vs the CPU oracle,
+per instance, bit-exact."""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import dawn
+
+pytestmark = pytest.mark.gpu
+wl = importlib.import_module("gal_dawn_amd.workloads")
+
+
+def check_batch(orc, graphs, src, spg, weighted):
+    devs = [dawn.Graph(h) for h in graphs]
+    rows, st = dawn.batch_mssp(devs, src, spg)
+    for i, h in enumerate(graphs):
+        s = src[i * spg:(i + 1) * spg]
+        odist, _, _, _ = orc.mssp(h.row_ptr, h.col, h.weight if weighted else None, s)
+        assert np.array_equal(rows[i], odist), (i, h.n)
+    for g in devs:
+        g.close()
+    return st
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_batch_small_uneven(orc, weighted):
+    print(f"case: 24 uneven graphs n in [1k, 20k], weighted={weighted}", flush=True)
+    graphs, src, _ = wl.uneven(24, seed=7, weighted=weighted, nmax=20000)
+    st = check_batch(orc, graphs, src, 8, weighted)
+    assert st["steps"] > 0
+
+
+def test_batch_isolated_and_duplicate_sources(orc):
+    h = dawn.HostGraph.from_edges(50, np.arange(0, 20, dtype=np.int32), np.arange(1, 21, dtype=np.int32))
+    h2 = dawn.HostGraph.er(300, 3.0, 3)
+    src = np.array([30, 0, 0, 5, 49, 2, 2, 2], np.int32)  # isolated 30/49, duplicates
+    src2 = dawn.gen_sources(300, 8, 3)
+    check_batch(orc, [h, h2], np.concatenate([src, src2]), 8, False)
+
+
+def test_batch_c4_full_unweighted(orc):
+    print("case: 512 uneven graphs (seed 4), 8 sources each, unweighted", flush=True)
+    graphs, src, _ = wl.uneven(512, seed=4)
+    check_batch(orc, graphs, src, 8, False)
+
+
+def test_batch_c4_full_weighted(orc):
+    print("case: 512 uneven graphs (seed 4), 8 sources each, int32 weights", flush=True)
+    graphs, src, _ = wl.uneven(512, seed=4, weighted=True)
+    check_batch(orc, graphs, src, 8, True)
