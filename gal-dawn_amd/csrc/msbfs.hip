@@ -67,6 +67,8 @@ struct MsArgs {
   uint64_t* seen;
   uint64_t* next;
   uint64_t* hist;
+  uint32_t* fbits;     // [2][fwords] 1 bit per vertex: in the frontier of a
+  int64_t fwords;      // level (any lane), slot = level parity
   int32_t* q;          // [2][qcap] frontier entries: vertex
   int32_t* qb;         // [2][qcap] first out-edge of the entry's chunk
   int64_t qcap;
@@ -94,6 +96,7 @@ struct MsWorkspace {
   int32_t n = 0;
   int64_t npad = 0;
   DevBuf<uint64_t> seen, next, hist, seen_bak;
+  DevBuf<uint32_t> fbits;
   DevBuf<int32_t> q, qb, distbuf;
   int64_t qcap = 0;
   DevBuf<LevelCounters> ring, ring_bak;
@@ -180,6 +183,8 @@ __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
     a.hist[v] = 0ull;  // level-0 slot
     a.next[v] = 0ull;
   }
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.fwords; w += gs)
+    a.fbits[w] = 0u;  // level-0 frontier bitmap
   if (blockIdx.x == 0 && threadIdx.x < kRing) a.ring[threadIdx.x] = LevelCounters{};
   if (blockIdx.x == 0 && threadIdx.x < kLanes) {
     o.sum[threadIdx.x] = 0;
@@ -198,11 +203,7 @@ __global__ void ms_seed_kernel(MsArgs a) {
     const uint64_t bit = 1ull << i;
     if (a.hist[v] == 0) {
       const int d = out_deg(a, v);
-      for (int k = 0; k < d; k += kPushChunk) {
-        a.q[c.qlen] = v;
-        a.qb[c.qlen] = a.row_ptr[v] + k;
-        c.qlen++;
-      }
+      a.fbits[v >> 5] |= 1u << (v & 31);
       c.fv += 1;
       c.fe += (unsigned)d;
       c.scout += (unsigned)d;
@@ -252,8 +253,8 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
 
   if (st.mode == DAWN_MODE_PUSH) {
-    const int32_t* q_in = a.q + (int64_t)((lv - 1) & 1) * a.qcap;
-    const int32_t* qb_in = a.qb + (int64_t)((lv - 1) & 1) * a.qcap;
+    const int32_t* q_in = a.q;  // built by ms_compact_kernel for this step
+    const int32_t* qb_in = a.qb;
     const unsigned qn = a.ring[(lv - 1) % kRing].qlen;
     for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
       const int64_t i = c0 + threadIdx.x;
@@ -282,12 +283,13 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   } else if (st.mode == DAWN_MODE_BITMAP) {
     // Dense top-down: light frontier vertices one lane each, heavy ones in
     // out-edge chunks of kPullChunk edges, one wave per chunk.
+    const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
       const int32_t u = (int32_t)(base + lane);
-      const uint64_t f = u < a.n ? prev[u] : 0ull;
-      if (!f) continue;
+      if (u >= a.n || !((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
+      const uint64_t f = prev[u];
       const int beg = a.row_ptr[u];
       const int deg = a.row_ptr[u + 1] - beg;
       if (deg >= kHeavyDegree) continue;
@@ -314,6 +316,9 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     // Only lanes alive in the previous level can arrive now: a vertex looks
     // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
     const uint64_t alive = a.ring[(lv - 1) % kRing].alive;
+    // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
+    // a 64-bit word is gathered only for in-neighbours that are in it.
+    const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
     // Light vertices (in-degree < kHeavyDegree): one lane each, four gathers
     // in flight between early-exit checks; the lane owns next[v].
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
@@ -330,11 +335,17 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       int j = 0;
       for (; j + 4 <= deg; j += 4) {
         const int32_t u0 = nb[j], u1 = nb[j + 1], u2 = nb[j + 2], u3 = nb[j + 3];
-        acc_w |= prev[u0] | prev[u1] | prev[u2] | prev[u3];
+        if ((fb[u0 >> 5] >> (u0 & 31)) & 1u) acc_w |= prev[u0];
+        if ((fb[u1 >> 5] >> (u1 & 31)) & 1u) acc_w |= prev[u1];
+        if ((fb[u2 >> 5] >> (u2 & 31)) & 1u) acc_w |= prev[u2];
+        if ((fb[u3 >> 5] >> (u3 & 31)) & 1u) acc_w |= prev[u3];
         if ((acc_w & miss) == miss) break;
       }
       if ((acc_w & miss) != miss)
-        for (; j < deg; j++) acc_w |= prev[nb[j]];
+        for (; j < deg; j++) {
+          const int32_t u = nb[j];
+          if ((fb[u >> 5] >> (u & 31)) & 1u) acc_w |= prev[u];
+        }
       if (acc_w & miss) a.next[v] = acc_w & miss;
     }
     // Heavy vertices: their in-edge lists are cut into chunks of at most
@@ -348,7 +359,10 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       if ((a.next[v] & miss) == miss) continue;  // other chunks already found them all
       const int he = a.hc_end[c];
       uint64_t part = 0;
-      for (int k = a.hc_beg[c] + lane; k < he; k += kWave) part |= prev[a.in_col[k]];
+      for (int k = a.hc_beg[c] + lane; k < he; k += kWave) {
+        const int32_t u = a.in_col[k];
+        if ((fb[u >> 5] >> (u & 31)) & 1u) part |= prev[u];
+      }
       part = wave_or_u64(part) & miss;
       if (lane == 0 && part) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)part);
     }
@@ -357,56 +371,76 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
 
 // Folds next[] into the level slot, seen, the queue and the counters.
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
-  __shared__ int s_wtot[kBlock / kWave];
-  __shared__ unsigned s_base;
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
   uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad;
-  int32_t* q_out = a.q + (int64_t)(lv & 1) * a.qcap;
-  int32_t* qb_out = a.qb + (int64_t)(lv & 1) * a.qcap;
+  uint32_t* fb = a.fbits + (int64_t)(lv & 1) * a.fwords;
   LevelCounters* c_out = a.ring + (lv % kRing);
   const uint64_t alive_prev = a.ring[(lv - 1) % kRing].alive;
   MsAcc acc;
-  // Block-uniform loop: every thread of a workgroup runs the same trips.
   for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.npad; b0 += (int64_t)gridDim.x * kBlock) {
     const int32_t v = (int32_t)(b0 + threadIdx.x);
-    const bool valid = v < a.n;
-    uint64_t sv = ~0ull, nw = 0;
-    if (valid) {
+    uint64_t nw = 0;
+    if (v < a.n) {
       const uint64_t nx = a.next[v];
-      sv = a.seen[v];
+      const uint64_t sv = a.seen[v];
       if (nx) a.next[v] = 0ull;
       nw = nx & ~sv;
       cur[v] = nw;
-      if (nw) a.seen[v] = sv | nw;
-      if ((~(sv | nw) & alive_prev) != 0) {  // still misses a live lane
-        acc.uv += 1;
-        acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
-      }
-    }
-    const bool isf = valid && nw != 0;
-    if (__syncthreads_or(isf)) {
-      // One queue entry per kPushChunk out-edges of each frontier vertex,
-      // so a push step's work per entry is bounded whatever the degree.
-      const int d = isf ? out_deg(a, v) : 0;
-      const int k = (d + kPushChunk - 1) / kPushChunk;
-      const int pos = block_append_n(&c_out->qlen, k, s_wtot, &s_base);
-      if (isf) {
-        const int32_t rb = a.row_ptr[v];
-        for (int i = 0; i < k; i++) {
-          q_out[pos + i] = v;
-          qb_out[pos + i] = rb + i * kPushChunk;
-        }
+      if (nw) {
+        a.seen[v] = sv | nw;
+        const int d = out_deg(a, v);
         acc.fv += 1;
         acc.fe += (unsigned)d;
         acc.scout += (unsigned)(d > 0 ? d : 1);
         acc.lanes += (unsigned)__popcll(nw);
         acc.alive |= nw;
       }
+      if ((~(sv | nw) & alive_prev) != 0) {  // still misses a live lane
+        acc.uv += 1;
+        acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+      }
+    }
+    // Frontier bitmap of the level: each wave owns its 64 vertices' bits.
+    const uint64_t fm = __ballot(nw != 0);
+    if (lane_id() == 0) {
+      const int64_t w = (b0 + (threadIdx.x & ~(kWave - 1))) >> 5;
+      fb[w] = (uint32_t)fm;
+      fb[w + 1] = (uint32_t)(fm >> 32);
     }
   }
   acc.flush(c_out);
+}
+
+// Builds the frontier queue of the previous level from its bitmap, only
+// for a PUSH step: one entry per kPushChunk out-edges of each frontier
+// vertex, so the push step's work per entry is bounded whatever the degree.
+__global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
+  __shared__ int s_wtot[kBlock / kWave];
+  __shared__ unsigned s_base;
+  const SchedState st = *a.st;
+  if (st.done || st.mode != DAWN_MODE_PUSH) return;
+  const int lv = st.step;
+  const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
+  int32_t* q = a.q;
+  int32_t* qb = a.qb;
+  unsigned* qlen = &a.ring[(lv - 1) % kRing].qlen;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.npad; b0 += (int64_t)gridDim.x * kBlock) {
+    const int32_t v = (int32_t)(b0 + threadIdx.x);
+    const bool isf = v < a.n && ((fb[v >> 5] >> (v & 31)) & 1u);
+    if (!__syncthreads_or(isf)) continue;  // block-uniform
+    const int d = isf ? out_deg(a, v) : 0;
+    const int k = (d + kPushChunk - 1) / kPushChunk;
+    const int pos = block_append_n(qlen, k, s_wtot, &s_base);
+    if (isf) {
+      const int32_t rb = a.row_ptr[v];
+      for (int i = 0; i < k; i++) {
+        q[pos + i] = v;
+        qb[pos + i] = rb + i * kPushChunk;
+      }
+    }
+  }
 }
 
 // Turns levels [lo, hi) of the history into distances.  from_state: lo and
@@ -464,10 +498,11 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
 #pragma unroll
         for (int k = 0; k < kSlices; k++)
           if (k < K) d += (int32_t)((S[k] >> b) & 1ull) << k;
+        // Streaming output: non-temporal stores keep it out of the caches.
         if ((F >> b) & 1ull)
-          col0[(int64_t)b * stride] = d;
+          __builtin_nontemporal_store(d, col0 + (int64_t)b * stride);
         else if (fin && !((sv >> b) & 1ull))
-          col0[(int64_t)b * stride] = kInf;
+          __builtin_nontemporal_store(kInf, col0 + (int64_t)b * stride);
       }
     }
     // Per-lane sums: sum_b = lo * reach_b + sum_k 2^k * |{v : bit b of S_k}|.
@@ -510,6 +545,7 @@ int ensure_ms(dawn_graph* g) {
   ws->npad = ((int64_t)g->n + 63) / 64 * 64;
   DAWN_TRY(ws->seen.alloc(ws->npad));
   DAWN_TRY(ws->next.alloc(ws->npad));
+  DAWN_TRY(ws->fbits.alloc((size_t)2 * (ws->npad / 32)));
   DAWN_TRY(ws->hist.alloc((size_t)kHist * ws->npad));
   // Queue entries per level: one per kPushChunk out-edges of each frontier
   // vertex, at most n + m / kPushChunk.
@@ -544,6 +580,8 @@ MsArgs make_args(dawn_graph* g) {
   a.in_col = g->in_col;
   a.seen = ws->seen.p;
   a.next = ws->next.p;
+  a.fbits = ws->fbits.p;
+  a.fwords = ws->npad / 32;
   a.hist = ws->hist.p;
   a.q = ws->q.p;
   a.qb = ws->qb.p;
@@ -637,6 +675,7 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     ms_seed_kernel<<<1, 64, 0, s>>>(a);
     for (int i = 0; i < kChunk; i++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+      ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
       ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
       ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
     }
@@ -679,6 +718,7 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       }
       for (int i = 0; i < kChunk; i++) {
         ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+        ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
         ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
         ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
       }
@@ -785,6 +825,7 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
           DAWN_HIP(hipMemcpyAsync(ws->ring.p, ws->ring_bak.p, sizeof(LevelCounters) * kRing,
                                   hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipEventRecord(ws->e_a, s));
+          ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
           ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
           ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
           DAWN_HIP(hipEventRecord(ws->e_b, s));
@@ -863,7 +904,10 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
         break;
       }
       const int slot = st.mode == DAWN_MODE_PUSH ? 3 : st.mode == DAWN_MODE_PULL ? 4 : 5;
-      DAWN_TRY(timed(slot, [&] { ms_expand_kernel<<<grid, kBlock, 0, s>>>(a); }));
+      DAWN_TRY(timed(slot, [&] {
+        ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
+        ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+      }));
       DAWN_TRY(timed(6, [&] { ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a); }));
       if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
         DAWN_TRY(timed(7, [&] { ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
