@@ -119,6 +119,7 @@ def _sig(lib):
         "dawn_apsp_block": (c.c_int, [P, c.c_int32, c.c_int32, P, _I64P, c.c_int32, c.POINTER(Stats)]),
         "dawn_closeness": (c.c_int, [P, _I32P, c.c_int32, _F64P, _I64P, _F32P, c.c_int32, c.POINTER(Stats)]),
         "dawn_batch_mssp": (c.c_int, [c.POINTER(P), c.c_int32, _I32P, c.c_int32, P, _I64P, c.POINTER(Stats)]),
+        "dawn_probe_copy": (c.c_int, [c.c_int64, c.c_int32, _F64P]),
         "dawn_mssp_profile": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F64P, _I64P, _I32P]),
         "dawn_sched_set_mlp": (c.c_int, [_F32P, _F32P, _F32P]),
         "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, _I32P, c.c_int32,
@@ -305,6 +306,13 @@ class HostGraph:
         if self._owned and _lib is not None:
             _lib.dawn_csr_free(ctypes.byref(self._csr))
             self._owned = False
+
+
+def probe_copy(nbytes: int, iters: int = 5) -> float:
+    """Achieved GB/s (read + write) of a plain device copy of nbytes."""
+    g = ctypes.c_double(0)
+    _check(lib().dawn_probe_copy(nbytes, iters, ctypes.byref(g)), "dawn_probe_copy")
+    return g.value
 
 
 def gen_sources(n: int, count: int, seed: int) -> np.ndarray:

@@ -247,7 +247,6 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   if (st.done) return;
   const int lv = st.step;
   const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad;
-  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad;
   const int lane = lane_id();
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
@@ -471,57 +470,106 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   int K = 0;
   while ((1 << K) < hi - lo) K++;
   const int lane = lane_id();
-  const int64_t wstride = (int64_t)gridDim.x * kBlock;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+  // Each lane owns 4 consecutive vertices (a wave 256), so a row store is
+  // 16 bytes per lane, 1 KB per wave.  With one flush of every level
+  // (lo == 0, final) every (lane, vertex) is written: level or INF.
+  const bool vec_ok = dist && (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0) && lo == 0 && fin;
+  const int64_t wstride = (int64_t)gridDim.x * kBlock * 4;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock * 4 + (threadIdx.x & ~(kWave - 1)) * 4; base < a.n;
        base += wstride) {
-    const int32_t v = (int32_t)(base + lane);
-    const bool valid = v < a.n;
-    uint64_t F = 0, S[kSlices];
+    const int64_t v0 = base + 4 * lane;
+    uint64_t F[4], S[4][kSlices], sv[4];
 #pragma unroll
-    for (int k = 0; k < kSlices; k++) S[k] = 0;
-    uint64_t sv = 0;
-    if (valid) {
-      sv = a.seen[v];
+    for (int j = 0; j < 4; j++) {
+      F[j] = 0;
+      sv[j] = 0;
+#pragma unroll
+      for (int k = 0; k < kSlices; k++) S[j][k] = 0;
+    }
+    if (v0 + 3 < a.n) {
+      const ulonglong2* sp = reinterpret_cast<const ulonglong2*>(a.seen + v0);
+      const ulonglong2 s01 = sp[0], s23 = sp[1];
+      sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
       for (int l = lo; l < hi; l++) {
-        const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad + v];
+        const ulonglong2* hp = reinterpret_cast<const ulonglong2*>(a.hist + (int64_t)(l % a.H) * a.npad + v0);
+        const ulonglong2 h01 = hp[0], h23 = hp[1];
+        const uint64_t w[4] = {h01.x, h01.y, h23.x, h23.y};
         const int r = l - lo;
-        F |= w;
 #pragma unroll
-        for (int k = 0; k < kSlices; k++)
-          if ((r >> k) & 1) S[k] |= w;
+        for (int j = 0; j < 4; j++) {
+          F[j] |= w[j];
+#pragma unroll
+          for (int k = 0; k < kSlices; k++)
+            if ((r >> k) & 1) S[j][k] |= w[j];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int64_t v = v0 + j;
+        if (v >= a.n) continue;
+        sv[j] = a.seen[v];
+        for (int l = lo; l < hi; l++) {
+          const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad + v];
+          const int r = l - lo;
+          F[j] |= w;
+#pragma unroll
+          for (int k = 0; k < kSlices; k++)
+            if ((r >> k) & 1) S[j][k] |= w;
+        }
       }
     }
-    if (dist && valid) {
-      int32_t* col0 = dist + v;
+    if (dist && v0 < a.n) {
+      const bool full = vec_ok && v0 + 3 < a.n;
       for (int b = 0; b < nl; b++) {
-        int32_t d = lo;
+        int32_t d[4];
+        bool wr[4];
 #pragma unroll
-        for (int k = 0; k < kSlices; k++)
-          if (k < K) d += (int32_t)((S[k] >> b) & 1ull) << k;
-        // Streaming output: non-temporal stores keep it out of the caches.
-        if ((F >> b) & 1ull)
-          __builtin_nontemporal_store(d, col0 + (int64_t)b * stride);
-        else if (fin && !((sv >> b) & 1ull))
-          __builtin_nontemporal_store(kInf, col0 + (int64_t)b * stride);
+        for (int j = 0; j < 4; j++) {
+          int32_t x = lo;
+#pragma unroll
+          for (int k = 0; k < kSlices; k++)
+            if (k < K) x += (int32_t)((S[j][k] >> b) & 1ull) << k;
+          wr[j] = true;
+          if (!((F[j] >> b) & 1ull)) {
+            x = kInf;
+            wr[j] = fin && !((sv[j] >> b) & 1ull);
+          }
+          d[j] = x;
+        }
+        int32_t* row = dist + (int64_t)b * stride + v0;
+        if (full) {
+          *reinterpret_cast<int4*>(row) = make_int4(d[0], d[1], d[2], d[3]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (v0 + j < a.n && wr[j]) row[j] = d[j];
+        }
       }
     }
-    // Per-lane sums: sum_b = lo * reach_b + sum_k 2^k * |{v : bit b of S_k}|.
-    const uint64_t tf = wave_transpose64(F);
-    const unsigned long long reach = (unsigned long long)__popcll(tf);
-    unsigned long long sum = reach * (unsigned long long)lo;
+    // Per-lane sums: sum_b = lo * reach_b + sum_k 2^k * |{v : bit b of S_k}|,
+    // counted with 64x64 bit transposes (lane b receives lane-b bits).
+    unsigned long long sum = 0, reach = 0, edges = 0;
 #pragma unroll
-    for (int k = 0; k < kSlices; k++) {
-      if (k < K) {
-        const uint64_t t = wave_transpose64(S[k]);
-        sum += (unsigned long long)__popcll(t) << k;
+    for (int j = 0; j < 4; j++) {
+      const uint64_t tf = wave_transpose64(F[j]);
+      const unsigned long long rc = (unsigned long long)__popcll(tf);
+      reach += rc;
+      sum += rc * (unsigned long long)lo;
+#pragma unroll
+      for (int k = 0; k < kSlices; k++) {
+        if (k < K) {
+          const uint64_t t = wave_transpose64(S[j][k]);
+          sum += (unsigned long long)__popcll(t) << k;
+        }
       }
-    }
-    unsigned long long edges = 0;
-    if (o.want_edges) {
-      const int d = valid ? out_deg(a, v) : 0;
-      for (int i = 0; i < kWave; i++) {
-        const int di = __shfl(d, i);
-        if ((tf >> i) & 1ull) edges += (unsigned)di;
+      if (o.want_edges) {
+        const int64_t v = v0 + j;
+        const int dg = v < a.n ? out_deg(a, (int32_t)v) : 0;
+        for (int i = 0; i < kWave; i++) {
+          const int di = __shfl(dg, i);
+          if ((tf >> i) & 1ull) edges += (unsigned)di;
+        }
       }
     }
     if (sum) atomicAdd(&s_sum[lane], sum);
@@ -618,6 +666,10 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
 
 // Dense step kernels: at most 8 workgroups per CU (2048), grid-stride beyond.
 unsigned ms_grid(const dawn_graph* g) { return grid_for(std::max<int64_t>(g->n, 1), kBlock, 2048); }
+// Distance pass: 4 vertices per thread.
+unsigned ms_final_grid(const dawn_graph* g) {
+  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4, 2048);
+}
 
 // Stages one batch's sources (taken mod n) into the pinned batch block.
 void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
@@ -655,6 +707,7 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
+  const unsigned fgrid = ms_final_grid(g);
   unsigned long long* ls = ws->lane_stats.p;
 
   // Graph of one batch: fill, seed, kChunk steps, final pass (no-op unless
@@ -679,7 +732,7 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
       ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
     }
-    ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
+    ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
     DAWN_HIP(hipStreamEndCapture(s, &graph));
     hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -713,7 +766,7 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       MsOut o{ls, ls + kLanes, ls + 2 * kLanes, want_edges};
       const int stp = ws->h_state->step;
       if (stp > lo) {
-        ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, stp, 0);
+        ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, stp, 0);
         lo = stp;
       }
       for (int i = 0; i < kChunk; i++) {
@@ -726,7 +779,7 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
       DAWN_HIP(hipStreamSynchronize(s));
       if (ws->h_state->done) {
-        ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels, 1);
+        ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels, 1);
         break;
       }
       if (guard_steps > (int64_t)g->n + 2 * kChunk) return DAWN_ERR_HIP;
@@ -865,6 +918,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
+  const unsigned fgrid = ms_final_grid(g);
   unsigned long long* ls = ws->lane_stats.p;
   MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 0};
   for (int i = 0; i < 8; i++) {
@@ -899,7 +953,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
       DAWN_HIP(hipMemcpy(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost));
       const SchedState st = *ws->h_state;
       if (st.done) {
-        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, st.levels, 1); }));
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.levels, 1); }));
         if (levels_out) levels_out[batch] = st.levels;
         break;
       }
@@ -910,7 +964,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
       }));
       DAWN_TRY(timed(6, [&] { ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a); }));
       if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
-        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<grid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
         lo = st.step;
       }
     }

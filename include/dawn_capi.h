@@ -202,6 +202,12 @@ int dawn_mssp_profile(const dawn_graph* g, const int32_t* sources,
                       int32_t count, int32_t policy, double* kernel_ms,
                       int64_t* launches, int32_t* levels);
 
+/* Plain device-to-device copy of `bytes` (read + write), `iters` timed
+ * launches: achieved GB/s.  No counterpart in the reference; used to
+ * calibrate rocprof traffic counters and as the box's achievable HBM
+ * bandwidth next to the 8 TB/s peak. */
+int dawn_probe_copy(int64_t bytes, int32_t iters, double* gbps);
+
 /* ----------------------------------------------------------- scheduler --- */
 
 /* Number of per-step features read by the scheduler MLP. */
