@@ -57,7 +57,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(sources shard with no communication; see DESIGN.md)"}))
+                          "(sources shard across GPUs with no communication; run one process per GPU)"}))
         sys.exit(2)
 
     dawn = __graft_entry__.load_package()
@@ -156,7 +156,7 @@ def main():
     L = float(levels.mean())
     total_prof = sum(v["total_ms"] for v in kern.values())
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    # Algorithmic bytes per launch of each kernel.
+    # Algorithmic bytes per launch of each kernel: the bytes the step must move at least.
     per_launch = {
         "final": 8 * n * (L + 1) + 256 * n,           # seen + L level words in, 64 int32 out
         "finalize": 24 * n,                           # next + seen in, level word out

@@ -488,6 +488,19 @@ def batch_mssp(graphs: Sequence[Graph], sources, count_per_graph: int, want_dist
 
 # ---- scheduler network --------------------------------------------------------
 
+def tune(graph: "Graph", verbose: bool = False):
+    """Autotune the launch knobs on `graph` and cache them for its class."""
+    _check(lib().dawn_tune(graph._h, int(verbose)), "dawn_tune")
+
+
+def tune_load(path: str):
+    _check(lib().dawn_tune_load(path.encode()), "dawn_tune_load")
+
+
+def tune_save(path: str):
+    _check(lib().dawn_tune_save(path.encode()), "dawn_tune_save")
+
+
 def sched_set_mlp(params: Optional[np.ndarray], mean=None, inv_std=None):
     if params is None:
         _check(lib().dawn_sched_set_mlp(None, None, None), "dawn_sched_set_mlp")

@@ -71,6 +71,14 @@ struct DevBuf {
   }
 };
 
+// Launch knobs chosen by the autotuner (tune.hip) per graph class.
+struct TuneKnobs {
+  int32_t dense_blocks = 2048;  // grid cap of the batched dense step kernels
+  int32_t heavy_degree = 32;    // in/out-degree from which edge lists are chunked
+  int32_t pull_chunk = 256;     // edges per heavy chunk (one wave each)
+  int32_t ss_chunk = 0;         // single-source steps per captured graph (0: auto)
+};
+
 struct MsWorkspace;   // 64-lane batched engine state (msbfs.hip)
 struct SsWorkspace;   // single-source engines state (bfs.hip / sssp.hip)
 
@@ -106,6 +114,8 @@ struct dawn_graph {
   int32_t* oc_end = nullptr;
   int64_t num_oc = 0;
   std::vector<int32_t> h_row_ptr; // host copy of offsets (degrees for drivers)
+  std::vector<int32_t> h_in_row_ptr;  // host CSC offsets (directed graphs)
+  dawn::TuneKnobs knobs;
   hipStream_t stream = nullptr;
   std::mutex lock;                // one traversal per graph at a time
   dawn::MsWorkspace* ms = nullptr;
@@ -271,6 +281,13 @@ inline unsigned grid_for(int64_t items, int per_block = kBlock, unsigned cap = 8
 
 // Pointer kind of a caller buffer.
 bool is_device_pointer(const void* p);
+
+// (Re)builds the heavy chunk lists of g from g->knobs (graph_dev.hip).
+int build_chunks(dawn_graph* g);
+// Drops captured step graphs (their launch shapes depend on the knobs).
+void invalidate_graphs(dawn_graph* g);
+// Knobs cached for the graph's class, if any (tune.hip).
+bool tuned_knobs_for(const dawn_graph* g, TuneKnobs* out);
 
 // Workspace release hooks (defined by the engines).
 void ms_workspace_free(MsWorkspace* ws);

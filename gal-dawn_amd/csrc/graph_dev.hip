@@ -43,6 +43,67 @@ bool is_device_pointer(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+namespace {
+int upload_i32(int32_t** dst, const std::vector<int32_t>& src) {
+  hipError_t e = hipMalloc((void**)dst, sizeof(int32_t) * std::max<size_t>(src.size(), 1));
+  if (e != hipSuccess) {
+    *dst = nullptr;
+    set_hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    return e == hipErrorOutOfMemory ? DAWN_ERR_OUT_OF_MEMORY : DAWN_ERR_HIP;
+  }
+  if (!src.empty()) DAWN_HIP(hipMemcpy(*dst, src.data(), sizeof(int32_t) * src.size(), hipMemcpyHostToDevice));
+  return DAWN_OK;
+}
+
+void free_chunks(dawn_graph* g) {
+  if (g->oc_v && g->oc_v != g->hc_v) {
+    (void)hipFree(g->oc_v);
+    (void)hipFree(g->oc_beg);
+    (void)hipFree(g->oc_end);
+  }
+  if (g->hc_v) (void)hipFree(g->hc_v);
+  if (g->hc_beg) (void)hipFree(g->hc_beg);
+  if (g->hc_end) (void)hipFree(g->hc_end);
+  g->hc_v = g->hc_beg = g->hc_end = g->oc_v = g->oc_beg = g->oc_end = nullptr;
+  g->num_hc = g->num_oc = 0;
+}
+}  // namespace
+
+// Heavy in-edge chunks for the pull step (dawn_graph::hc_v) and heavy
+// out-edge chunks for the dense top-down step, from g->knobs.
+int build_chunks(dawn_graph* g) {
+  free_chunks(g);
+  const int heavy = g->knobs.heavy_degree, chunk = g->knobs.pull_chunk;
+  auto chunks = [&](const std::vector<int32_t>& rp, int32_t** dv, int32_t** db, int32_t** de,
+                    int64_t* count) -> int {
+    std::vector<int32_t> hv, hb, he;
+    for (int32_t v = 0; v < g->n; v++) {
+      const int32_t b = rp[v], e = rp[v + 1];
+      if (e - b < heavy) continue;
+      for (int32_t k = b; k < e; k += chunk) {
+        hv.push_back(v);
+        hb.push_back(k);
+        he.push_back(std::min(e, k + chunk));
+      }
+    }
+    *count = (int64_t)hv.size();
+    DAWN_TRY(upload_i32(dv, hv));
+    DAWN_TRY(upload_i32(db, hb));
+    return upload_i32(de, he);
+  };
+  DAWN_TRY(chunks(g->directed ? g->h_in_row_ptr : g->h_row_ptr, &g->hc_v, &g->hc_beg, &g->hc_end,
+                  &g->num_hc));
+  if (g->directed) {
+    DAWN_TRY(chunks(g->h_row_ptr, &g->oc_v, &g->oc_beg, &g->oc_end, &g->num_oc));
+  } else {
+    g->oc_v = g->hc_v;
+    g->oc_beg = g->hc_beg;
+    g->oc_end = g->hc_end;
+    g->num_oc = g->num_hc;
+  }
+  return DAWN_OK;
+}
+
 }  // namespace dawn
 
 using namespace dawn;
@@ -133,40 +194,9 @@ int dawn_graph_create(const dawn_csr* csr, dawn_graph** out) {
     if ((rc = up(&g->in_col, ic.data(), (size_t)m))) return fail(rc);
     if (csr->weight && (rc = up(&g->in_weight, iw.data(), (size_t)m))) return fail(rc);
   }
-  {
-    // Heavy in-edge chunks for the pull step (see dawn_graph::hc_v) and
-    // heavy out-edge chunks for the dense top-down step.
-    auto chunks = [&](const std::vector<int32_t>& rp, int32_t** dv, int32_t** db, int32_t** de,
-                      int64_t* count) -> int {
-      std::vector<int32_t> hv, hb, he;
-      for (int32_t v = 0; v < n; v++) {
-        const int32_t b = rp[v], e = rp[v + 1];
-        if (e - b < kHeavyDegree) continue;
-        for (int32_t k = b; k < e; k += kPullChunk) {
-          hv.push_back(v);
-          hb.push_back(k);
-          he.push_back(std::min(e, k + kPullChunk));
-        }
-      }
-      *count = (int64_t)hv.size();
-      int r;
-      if ((r = up(dv, hv.data(), hv.size()))) return r;
-      if ((r = up(db, hb.data(), hb.size()))) return r;
-      return up(de, he.data(), he.size());
-    };
-    if ((rc = chunks(g->directed ? in_rp_host : g->h_row_ptr, &g->hc_v, &g->hc_beg, &g->hc_end,
-                     &g->num_hc)))
-      return fail(rc);
-    if (g->directed) {
-      if ((rc = chunks(g->h_row_ptr, &g->oc_v, &g->oc_beg, &g->oc_end, &g->num_oc)))
-        return fail(rc);
-    } else {
-      g->oc_v = g->hc_v;
-      g->oc_beg = g->hc_beg;
-      g->oc_end = g->hc_end;
-      g->num_oc = g->num_hc;
-    }
-  }
+  if (g->directed) g->h_in_row_ptr = in_rp_host;
+  (void)tuned_knobs_for(g, &g->knobs);
+  if ((rc = build_chunks(g))) return fail(rc);
   *out = g;
   return DAWN_OK;
 }
@@ -182,14 +212,7 @@ int dawn_graph_destroy(dawn_graph* g) {
     if (g->in_col) (void)hipFree(g->in_col);
     if (g->in_weight) (void)hipFree(g->in_weight);
   }
-  if (g->directed) {
-    if (g->oc_v) (void)hipFree(g->oc_v);
-    if (g->oc_beg) (void)hipFree(g->oc_beg);
-    if (g->oc_end) (void)hipFree(g->oc_end);
-  }
-  if (g->hc_v) (void)hipFree(g->hc_v);
-  if (g->hc_beg) (void)hipFree(g->hc_beg);
-  if (g->hc_end) (void)hipFree(g->hc_end);
+  free_chunks(g);
   if (g->row_ptr) (void)hipFree(g->row_ptr);
   if (g->col) (void)hipFree(g->col);
   if (g->weight) (void)hipFree(g->weight);

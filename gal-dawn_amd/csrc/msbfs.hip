@@ -80,6 +80,7 @@ struct MsArgs {
   const int32_t* oc_beg;
   const int32_t* oc_end;
   int64_t num_oc;
+  int32_t heavy;          // degree from which a vertex's edges are chunked
   LevelCounters* ring;
   SchedState* st;
   MsBatch* batch;
@@ -110,6 +111,18 @@ struct MsWorkspace {
   std::map<std::pair<long long, const void*>, hipGraphExec_t> graphs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, e_a = nullptr, e_b = nullptr;
 };
+
+void ss_invalidate_graphs(SsWorkspace* ws);
+
+// Captured step graphs bake grid sizes; the autotuner drops them when it
+// changes the knobs.
+void invalidate_graphs(dawn_graph* g) {
+  if (g->ms) {
+    for (auto& kv : g->ms->graphs) (void)hipGraphExecDestroy(kv.second);
+    g->ms->graphs.clear();
+  }
+  ss_invalidate_graphs(g->ss);
+}
 
 void ms_workspace_free(MsWorkspace* ws) {
   if (!ws) return;
@@ -281,7 +294,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     }
   } else if (st.mode == DAWN_MODE_BITMAP) {
     // Dense top-down: light frontier vertices one lane each, heavy ones in
-    // out-edge chunks of kPullChunk edges, one wave per chunk.
+    // out-edge chunks of knobs.pull_chunk edges, one wave per chunk.
     const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
@@ -291,7 +304,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       const uint64_t f = prev[u];
       const int beg = a.row_ptr[u];
       const int deg = a.row_ptr[u + 1] - beg;
-      if (deg >= kHeavyDegree) continue;
+      if (deg >= a.heavy) continue;
       for (int j = 0; j < deg; j++) {
         const int32_t v = a.col[beg + j];
         const uint64_t nv = f & ~a.seen[v];
@@ -318,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
     // a 64-bit word is gathered only for in-neighbours that are in it.
     const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
-    // Light vertices (in-degree < kHeavyDegree): one lane each, four gathers
+    // Light vertices (in-degree < a.heavy): one lane each, four gathers
     // in flight between early-exit checks; the lane owns next[v].
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
@@ -328,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       if (!miss) continue;
       const int beg = a.in_row_ptr[v];
       const int deg = a.in_row_ptr[v + 1] - beg;
-      if (deg >= kHeavyDegree || deg == 0) continue;
+      if (deg >= a.heavy || deg == 0) continue;
       const int32_t* nb = a.in_col + beg;
       uint64_t acc_w = 0;
       int j = 0;
@@ -348,7 +361,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       if (acc_w & miss) a.next[v] = acc_w & miss;
     }
     // Heavy vertices: their in-edge lists are cut into chunks of at most
-    // kPullChunk edges, one wave per chunk, merged with atomicOr.
+    // knobs.pull_chunk edges, one wave per chunk, merged with atomicOr.
     const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t c = wave; c < a.num_hc; c += nwaves) {
@@ -642,6 +655,7 @@ MsArgs make_args(dawn_graph* g) {
   a.oc_beg = g->oc_beg;
   a.oc_end = g->oc_end;
   a.num_oc = g->num_oc;
+  a.heavy = g->knobs.heavy_degree;
   a.ring = ws->ring.p;
   a.st = ws->st.p;
   a.batch = ws->batch.p;
@@ -665,10 +679,12 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
 }
 
 // Dense step kernels: at most 8 workgroups per CU (2048), grid-stride beyond.
-unsigned ms_grid(const dawn_graph* g) { return grid_for(std::max<int64_t>(g->n, 1), kBlock, 2048); }
+unsigned ms_grid(const dawn_graph* g) {
+  return grid_for(std::max<int64_t>(g->n, 1), kBlock, (unsigned)g->knobs.dense_blocks);
+}
 // Distance pass: 4 vertices per thread.
 unsigned ms_final_grid(const dawn_graph* g) {
-  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4, 2048);
+  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4, (unsigned)g->knobs.dense_blocks);
 }
 
 // Stages one batch's sources (taken mod n) into the pinned batch block.

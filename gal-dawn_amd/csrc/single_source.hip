@@ -63,6 +63,12 @@ struct SsWorkspace {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+void ss_invalidate_graphs(SsWorkspace* ws) {
+  if (!ws) return;
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  ws->graphs.clear();
+}
+
 void ss_workspace_free(SsWorkspace* ws) {
   if (!ws) return;
   for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -603,7 +609,7 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
 
   const unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock, 4096);
   // A chunk = K decide+expand pairs, captured once per (kind, policy, mode, K).
-  int K = g->n < (1 << 20) ? 8 : 16;
+  int K = g->knobs.ss_chunk > 0 ? g->knobs.ss_chunk : (g->n < (1 << 20) ? 8 : 16);
   if (const char* e = std::getenv("DAWN_SS_CHUNK")) K = std::max(1, std::atoi(e));
   // Launch arguments are baked into the graph: key it by every one that
   // varies between calls (the network pointer changes when one is installed).

@@ -1,0 +1,192 @@
+// Autotuner for the engine's launch knobs (no counterpart in the reference,
+// whose launch shape is fixed at block_size=1024 per vertex thread,
+// src/algorithm/gpu/bfs.cu:67-68).
+//
+// Knobs (dawn::TuneKnobs): grid cap of the batched dense kernels, the degree
+// from which edge lists are cut into wave chunks, the chunk length, and the
+// single-source steps per captured hipGraph.  dawn_tune sweeps them on a
+// graph (coordinate descent, HIP-event device time of a sample traversal,
+// best of two runs), applies the winner to the graph and caches it for the
+// graph's class (log2 n, log2 average degree, directed, weighted); graphs
+// created later in that class start from the cached knobs.  The cache
+// persists as text: one line "class dense_blocks heavy_degree pull_chunk
+// ss_chunk".
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace dawn {
+
+int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats);
+int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out, int32_t policy,
+                      dawn_stats* stats, int32_t force_mode, float* trace_feat,
+                      int32_t* trace_mode, int32_t trace_max, int32_t* trace_rows);
+
+namespace {
+std::mutex g_tune_lock;
+std::map<std::string, TuneKnobs> g_tune_cache;
+
+std::string class_key(const dawn_graph* g) {
+  const int ln = g->n > 1 ? (int)std::lround(std::log2((double)g->n)) : 0;
+  const double avg = g->n ? (double)g->m / g->n : 0.0;
+  const int ld = avg > 0 ? (int)std::lround(std::log2(avg)) : 0;
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "n%d_d%d_%c%c", ln, ld, g->directed ? 'D' : 'U',
+                g->weighted ? 'W' : 'P');
+  return buf;
+}
+
+// Device time of a sample traversal with the graph's current knobs.
+int sample_ms(dawn_graph* g, const std::vector<int32_t>& src, double* ms) {
+  double best = 1e300;
+  for (int rep = 0; rep < 3; rep++) {  // the first run also warms up
+    dawn_stats st{};
+    if (g->weighted) {
+      for (int32_t s : src)
+        DAWN_TRY(run_single_source(g, 1, s, nullptr, DAWN_POLICY_FIXED, &st, -1, nullptr, nullptr,
+                                   0, nullptr));
+    } else {
+      DAWN_TRY(run_msbfs(g, src.data(), (int32_t)src.size(), nullptr, nullptr, nullptr, nullptr,
+                         DAWN_POLICY_FIXED, &st));
+    }
+    if (rep > 0) best = std::min(best, st.device_ms);
+  }
+  *ms = best;
+  return DAWN_OK;
+}
+
+int apply(dawn_graph* g, const TuneKnobs& k) {
+  std::lock_guard<std::mutex> guard(g->lock);  // traversals hold it while they run
+  if (g->stream) DAWN_HIP(hipStreamSynchronize(g->stream));
+  const bool rechunk = k.heavy_degree != g->knobs.heavy_degree || k.pull_chunk != g->knobs.pull_chunk;
+  g->knobs = k;
+  invalidate_graphs(g);
+  if (rechunk) {
+    (void)hipSetDevice(g->device);
+    DAWN_TRY(build_chunks(g));
+  }
+  return DAWN_OK;
+}
+}  // namespace
+
+bool tuned_knobs_for(const dawn_graph* g, TuneKnobs* out) {
+  std::lock_guard<std::mutex> l(g_tune_lock);
+  auto it = g_tune_cache.find(class_key(g));
+  if (it == g_tune_cache.end()) return false;
+  *out = it->second;
+  return true;
+}
+
+}  // namespace dawn
+
+using namespace dawn;
+
+extern "C" {
+
+int dawn_tune(const dawn_graph* gc, int32_t verbose) {
+  dawn_graph* g = const_cast<dawn_graph*>(gc);
+  if (!g || g->n <= 0) return DAWN_ERR_INVALID_ARG;
+  // Sample sources: the highest-degree vertex and spread vertices with edges.
+  std::vector<int32_t> src;
+  const int want = g->weighted ? 2 : 128;
+  for (int32_t v = 0; v < g->n && (int)src.size() < want; v += std::max(1, g->n / (4 * want)))
+    if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
+  if (src.empty()) return DAWN_ERR_INVALID_ARG;
+
+  TuneKnobs best = g->knobs;
+  double best_ms = 0;
+  DAWN_TRY(sample_ms(g, src, &best_ms));
+  auto consider = [&](TuneKnobs k) -> int {
+    DAWN_TRY(apply(g, k));
+    double ms = 0;
+    DAWN_TRY(sample_ms(g, src, &ms));
+    if (verbose)
+      std::fprintf(stderr, "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d: %.3f ms\n",
+                   class_key(g).c_str(), k.dense_blocks, k.heavy_degree, k.pull_chunk, k.ss_chunk,
+                   ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+    return DAWN_OK;
+  };
+  if (g->weighted) {
+    for (int c : {4, 8, 16, 32}) {
+      TuneKnobs k = best;
+      k.ss_chunk = c;
+      DAWN_TRY(consider(k));
+    }
+  } else {
+    for (int b : {512, 1024, 2048, 4096}) {
+      TuneKnobs k = best;
+      k.dense_blocks = b;
+      DAWN_TRY(consider(k));
+    }
+    const int pairs[][2] = {{16, 256}, {32, 128}, {32, 256}, {32, 512}, {64, 256}, {64, 1024}};
+    const TuneKnobs base = best;
+    for (auto& p : pairs) {
+      TuneKnobs k = base;
+      k.heavy_degree = p[0];
+      k.pull_chunk = p[1];
+      DAWN_TRY(consider(k));
+    }
+  }
+  DAWN_TRY(apply(g, best));
+  {
+    std::lock_guard<std::mutex> l(g_tune_lock);
+    g_tune_cache[class_key(g)] = best;
+  }
+  if (verbose)
+    std::fprintf(stderr, "tune %s -> blocks=%d heavy=%d chunk=%d ss_chunk=%d (%.3f ms)\n",
+                 class_key(g).c_str(), best.dense_blocks, best.heavy_degree, best.pull_chunk,
+                 best.ss_chunk, best_ms);
+  return DAWN_OK;
+}
+
+int dawn_tune_load(const char* path) {
+  if (!path) return DAWN_ERR_INVALID_ARG;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return DAWN_ERR_IO;
+  char key[64];
+  TuneKnobs k;
+  std::map<std::string, TuneKnobs> loaded;
+  int rc = DAWN_OK;
+  for (;;) {
+    const int got = std::fscanf(f, "%63s %d %d %d %d", key, &k.dense_blocks, &k.heavy_degree,
+                                &k.pull_chunk, &k.ss_chunk);
+    if (got == EOF) break;
+    if (got != 5 || k.dense_blocks < 1 || k.heavy_degree < 1 || k.pull_chunk < 1 || k.ss_chunk < 0) {
+      rc = DAWN_ERR_FORMAT;
+      break;
+    }
+    loaded[key] = k;
+  }
+  std::fclose(f);
+  if (rc != DAWN_OK) return rc;
+  std::lock_guard<std::mutex> l(g_tune_lock);
+  for (auto& kv : loaded) g_tune_cache[kv.first] = kv.second;
+  return DAWN_OK;
+}
+
+int dawn_tune_save(const char* path) {
+  if (!path) return DAWN_ERR_INVALID_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return DAWN_ERR_IO;
+  std::lock_guard<std::mutex> l(g_tune_lock);
+  for (auto& kv : g_tune_cache)
+    std::fprintf(f, "%s %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
+                 kv.second.heavy_degree, kv.second.pull_chunk, kv.second.ss_chunk);
+  std::fclose(f);
+  return DAWN_OK;
+}
+
+}  // extern "C"

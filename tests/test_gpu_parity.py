@@ -1,6 +1,6 @@
 """HIP engine vs the CPU oracle, bit-exact, through the C ABI.
 
-Sizes: the small cases run every mode/policy; the BASELINE configurations
+Sizes: the small cases run every mode/policy; the headline workloads
 (ER n=65,536 seed 1; 2048^2 grid + 1% long-range seed 2; RMAT scale 20
 seed 3) run at full size with the oracle on sampled sources plus
 size-independent checks (BFS level equations, symmetry of undirected
@@ -182,3 +182,26 @@ def test_mssp_c3_full_size_properties(orc):
     for i in range(0, 128, 16):
         for j in range(0, 128, 16):
             assert dist[i][sub[j]] == dist[j][sub[i]]
+
+
+def test_tuned_knobs_keep_results_exact(orc, tmp_path):
+    """Every knob setting the tuner can pick must leave results unchanged."""
+    log("case: tuner on RMAT scale 13 ef 16 seed 4 and weighted grid 96")
+    h = dawn.HostGraph.rmat(13, 16, 4)
+    g = dawn.Graph(h)
+    srcs = dawn.gen_sources(h.n, 96, 4)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    dawn.tune(g)
+    dist, sums = g.mssp(srcs)
+    assert np.array_equal(dist, odist) and np.array_equal(sums, osums)
+    # A graph created afterwards in the same class starts from the cache.
+    g2 = dawn.Graph(h)
+    assert np.array_equal(g2.mssp(srcs)[0], odist)
+    hw = dawn.HostGraph.grid(96, 0.01, 4, weighted=True)
+    gw = dawn.Graph(hw)
+    dawn.tune(gw)
+    od = orc.sssp(hw.row_ptr, hw.col, hw.weight, 5)
+    assert np.array_equal(gw.sssp(5), od)
+    p = str(tmp_path / "tune.txt")
+    dawn.tune_save(p)
+    assert len(open(p).read().splitlines()) >= 2

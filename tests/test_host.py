@@ -173,3 +173,17 @@ def test_no_device_calls_fail_cleanly():
     with pytest.raises(dawn.DawnError) as e:
         dawn.Graph(h)
     assert e.value.status == dawn.ERR_NO_DEVICE
+
+
+def test_tune_cache_roundtrip(tmp_path):
+    p = tmp_path / "tune.txt"
+    p.write_text("n20_d5_UP 1024 32 512 0\nn12_d3_UW 2048 16 256 8\n")
+    dawn.tune_load(str(p))
+    q = tmp_path / "out.txt"
+    dawn.tune_save(str(q))
+    lines = set(q.read_text().splitlines())
+    assert {"n20_d5_UP 1024 32 512 0", "n12_d3_UW 2048 16 256 8"} <= lines
+    bad = tmp_path / "bad.txt"
+    bad.write_text("n1 x\n")
+    with pytest.raises(dawn.DawnError):
+        dawn.tune_load(str(bad))
