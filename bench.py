@@ -14,6 +14,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -24,6 +25,7 @@ import __graft_entry__  # noqa: E402
 
 METRIC = "GTEPS + MSSP sources/s on 1 MI355X, learned vs fixed schedule, bit-exact dists"
 HBM_PEAK_GBS = 8000.0
+TUNE_PATH = os.path.join(ROOT, "gal-dawn_amd", "tune_cache.txt")
 
 
 def log(msg):
@@ -48,6 +50,9 @@ def main():
     ap.add_argument("--seed", type=int, default=99)
     ap.add_argument("--cpu-sample", type=int, default=0, help="sources for the CPU baseline (0: auto)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--retune", action="store_true",
+                    help="sweep the launch knobs on the graph instead of loading the tune file")
+    ap.add_argument("--tune-out", default=None, help="with --retune: write the tune cache here")
     ap.add_argument("--check", type=int, default=16, help="sources checked bit-exactly vs the oracle")
     ap.add_argument("--quick-profile", action="store_true",
                     help="one learned-schedule pass only (for rocprofv3 runs)")
@@ -73,10 +78,27 @@ def main():
     build_s = time.time() - t0
     n, m = h.n, h.m
     log(f"graph RMAT scale {args.scale} ef {args.edge_factor}: n={n} m={m} built in {build_s:.1f}s")
+    # Launch knobs measured by the autotuner on this graph class, committed
+    # beside the package; graphs created afterwards start from them.
+    if os.path.exists(TUNE_PATH) and not args.retune:
+        dawn.tune_load(TUNE_PATH)
     t0 = time.time()
     g = dawn.Graph(h)
     upload_s = time.time() - t0
     src = dawn.gen_sources(n, args.sources, args.seed)
+    tune_s = 0.0
+    if args.retune:
+        # Sweep the launch knobs for this graph class (untimed set-up).
+        t0 = time.time()
+        dawn.tune(g)
+        tune_s = time.time() - t0
+        if args.tune_out:
+            dawn.tune_save(args.tune_out)
+    with tempfile.NamedTemporaryFile("r", suffix=".txt") as tf:
+        dawn.tune_save(tf.name)
+        knobs = open(tf.name).read().split()
+    log(f"launch knobs (class dense_blocks heavy chunk ss_chunk balanced): {knobs or 'defaults'}"
+        f"{f' tuned in {tune_s:.1f}s' if args.retune else ''}")
 
     # Learned scheduler network (trained offline by gal-dawn_amd/train_sched.py).
     learned_loaded = False
@@ -249,6 +271,8 @@ def main():
         "parity": {"checked_sources": int(len(chk)) if args.check > 0 else 0, "dist_bit_exact": parity,
                    "timed_sums_bit_exact": parity_sums},
         "graph_build_s": build_s,
+        "tune_s": tune_s,
+        "knobs": knobs,
         "upload_s": upload_s,
         "cpu_baseline": cpu,
     }

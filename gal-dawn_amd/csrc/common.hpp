@@ -77,6 +77,7 @@ struct TuneKnobs {
   int32_t heavy_degree = 32;    // in/out-degree from which edge lists are chunked
   int32_t pull_chunk = 256;     // edges per heavy chunk (one wave each)
   int32_t ss_chunk = 0;         // single-source steps per captured graph (0: auto)
+  int32_t pull_balanced = 0;    // light pull: 1 = edges spread over the workgroup
 };
 
 struct MsWorkspace;   // 64-lane batched engine state (msbfs.hip)

@@ -81,6 +81,8 @@ struct MsArgs {
   const int32_t* oc_end;
   int64_t num_oc;
   int32_t heavy;          // degree from which a vertex's edges are chunked
+  int32_t balanced;       // light pull edge-balanced over the workgroup
+  int32_t deg_bits;       // bits of the largest out-degree
   LevelCounters* ring;
   SchedState* st;
   MsBatch* batch;
@@ -331,9 +333,46 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
     // a 64-bit word is gathered only for in-neighbours that are in it.
     const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    if (a.balanced) {
+      // Light vertices, edge-balanced: the workgroup's 256 vertices' in-edge
+      // lists are concatenated (degree scan) and read one edge per thread,
+      // coalesced; hits are OR-ed into the owner's LDS word.  Gathers stop
+      // for an owner once it has found all its missing lanes.
+      __shared__ unsigned long long s_acc[kBlock];
+      for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += wstride) {
+        const int32_t v = (int32_t)(b0 + threadIdx.x);
+        const uint64_t miss = v < a.n ? (~a.seen[v] & alive) : 0ull;
+        int beg = 0, deg = 0;
+        if (miss) {
+          beg = a.in_row_ptr[v];
+          const int d = a.in_row_ptr[v + 1] - beg;
+          deg = d < a.heavy ? d : 0;
+        }
+        int total;
+        const int off = block_exclusive_scan(deg, s_scan, &total);
+        s_off[threadIdx.x] = off;
+        s_beg[threadIdx.x] = beg;
+        s_f[threadIdx.x] = miss;
+        s_acc[threadIdx.x] = 0ull;
+        __syncthreads();
+        for (int e = threadIdx.x; e < total; e += kBlock) {
+          const int j = owner_of(s_off, e);
+          const int32_t u = a.in_col[s_beg[j] + (e - s_off[j])];
+          if ((fb[u >> 5] >> (u & 31)) & 1u) {
+            const uint64_t mj = s_f[j];
+            if ((__atomic_load_n(&s_acc[j], __ATOMIC_RELAXED) & mj) != mj) {
+              const uint64_t hit = prev[u] & mj;
+              if (hit) atomicOr(&s_acc[j], (unsigned long long)hit);
+            }
+          }
+        }
+        __syncthreads();
+        if (deg && s_acc[threadIdx.x]) a.next[v] = s_acc[threadIdx.x];
+      }
+    } else
     // Light vertices (in-degree < a.heavy): one lane each, four gathers
     // in flight between early-exit checks; the lane owns next[v].
-    const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
       const int32_t v = (int32_t)(base + lane);
@@ -517,7 +556,6 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         }
       }
     } else {
-#pragma unroll
       for (int j = 0; j < 4; j++) {
         const int64_t v = v0 + j;
         if (v >= a.n) continue;
@@ -563,9 +601,11 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
     // Per-lane sums: sum_b = lo * reach_b + sum_k 2^k * |{v : bit b of S_k}|,
     // counted with 64x64 bit transposes (lane b receives lane-b bits).
     unsigned long long sum = 0, reach = 0, edges = 0;
+    uint64_t tfs[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint64_t tf = wave_transpose64(F[j]);
+      tfs[j] = tf;
       const unsigned long long rc = (unsigned long long)__popcll(tf);
       reach += rc;
       sum += rc * (unsigned long long)lo;
@@ -576,12 +616,18 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
           sum += (unsigned long long)__popcll(t) << k;
         }
       }
-      if (o.want_edges) {
-        const int64_t v = v0 + j;
-        const int dg = v < a.n ? out_deg(a, (int32_t)v) : 0;
-        for (int i = 0; i < kWave; i++) {
-          const int di = __shfl(dg, i);
-          if ((tf >> i) & 1ull) edges += (unsigned)di;
+    }
+    if (o.want_edges) {
+      // edges_b = sum_i [bit i of tf] * deg_i, bit-sliced over the degree:
+      // sum_t 2^t * popcount(tf & ballot(bit t of deg)).
+      int dg[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) dg[j] = v0 + j < a.n ? out_deg(a, (int32_t)(v0 + j)) : 0;
+      for (int t = 0; t < a.deg_bits; t++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint64_t dm = __ballot((dg[j] >> t) & 1);
+          edges += (unsigned long long)__popcll(tfs[j] & dm) << t;
         }
       }
     }
@@ -656,6 +702,8 @@ MsArgs make_args(dawn_graph* g) {
   a.oc_end = g->oc_end;
   a.num_oc = g->num_oc;
   a.heavy = g->knobs.heavy_degree;
+  a.balanced = g->knobs.pull_balanced;
+  a.deg_bits = 32 - __builtin_clz((unsigned)std::max(1, g->max_out_degree));
   a.ring = ws->ring.p;
   a.st = ws->st.p;
   a.batch = ws->batch.p;
@@ -928,7 +976,6 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   DAWN_HIP(hipSetDevice(g->device));
   DAWN_TRY(ensure_ms(g));
   MsWorkspace* ws = g->ms;
-  DAWN_TRY(ws->distbuf.alloc((size_t)kLanes * g->n));
   hipStream_t s = g->stream;
   MsArgs a = make_args(g);
   SchedParams p = make_params(g, policy);
@@ -936,7 +983,8 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   const unsigned grid = ms_grid(g);
   const unsigned fgrid = ms_final_grid(g);
   unsigned long long* ls = ws->lane_stats.p;
-  MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 0};
+  // Same work as a sums-only traversal with statistics (no distance rows).
+  MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 1};
   for (int i = 0; i < 8; i++) {
     kernel_ms[i] = 0;
     launches[i] = 0;
@@ -957,8 +1005,8 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   for (int32_t b0 = 0; b0 < S; b0 += kLanes, batch++) {
     const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
     stage_batch(ws, sources + b0, lanes);
-    ws->h_batch->dist = ws->distbuf.p;
-    ws->h_batch->stride = g->n;
+    ws->h_batch->dist = nullptr;
+    ws->h_batch->stride = 0;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     DAWN_TRY(timed(0, [&] { ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o); }));
     DAWN_TRY(timed(1, [&] { ms_seed_kernel<<<1, 64, 0, s>>>(a); }));

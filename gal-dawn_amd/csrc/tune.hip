@@ -10,7 +10,7 @@
 // graph's class (log2 n, log2 average degree, directed, weighted); graphs
 // created later in that class start from the cached knobs.  The cache
 // persists as text: one line "class dense_blocks heavy_degree pull_chunk
-// ss_chunk".
+// ss_chunk pull_balanced".
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -110,9 +110,9 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     double ms = 0;
     DAWN_TRY(sample_ms(g, src, &ms));
     if (verbose)
-      std::fprintf(stderr, "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d: %.3f ms\n",
+      std::fprintf(stderr, "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d balanced=%d: %.3f ms\n",
                    class_key(g).c_str(), k.dense_blocks, k.heavy_degree, k.pull_chunk, k.ss_chunk,
-                   ms);
+                   k.pull_balanced, ms);
     if (ms < best_ms) {
       best_ms = ms;
       best = k;
@@ -126,6 +126,11 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       DAWN_TRY(consider(k));
     }
   } else {
+    for (int pb : {0, 1}) {
+      TuneKnobs k = best;
+      k.pull_balanced = pb;
+      DAWN_TRY(consider(k));
+    }
     for (int b : {512, 1024, 2048, 4096}) {
       TuneKnobs k = best;
       k.dense_blocks = b;
@@ -146,9 +151,9 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     g_tune_cache[class_key(g)] = best;
   }
   if (verbose)
-    std::fprintf(stderr, "tune %s -> blocks=%d heavy=%d chunk=%d ss_chunk=%d (%.3f ms)\n",
+    std::fprintf(stderr, "tune %s -> blocks=%d heavy=%d chunk=%d ss_chunk=%d balanced=%d (%.3f ms)\n",
                  class_key(g).c_str(), best.dense_blocks, best.heavy_degree, best.pull_chunk,
-                 best.ss_chunk, best_ms);
+                 best.ss_chunk, best.pull_balanced, best_ms);
   return DAWN_OK;
 }
 
@@ -161,10 +166,11 @@ int dawn_tune_load(const char* path) {
   std::map<std::string, TuneKnobs> loaded;
   int rc = DAWN_OK;
   for (;;) {
-    const int got = std::fscanf(f, "%63s %d %d %d %d", key, &k.dense_blocks, &k.heavy_degree,
-                                &k.pull_chunk, &k.ss_chunk);
+    const int got = std::fscanf(f, "%63s %d %d %d %d %d", key, &k.dense_blocks, &k.heavy_degree,
+                                &k.pull_chunk, &k.ss_chunk, &k.pull_balanced);
     if (got == EOF) break;
-    if (got != 5 || k.dense_blocks < 1 || k.heavy_degree < 1 || k.pull_chunk < 1 || k.ss_chunk < 0) {
+    if (got != 6 || k.dense_blocks < 1 || k.heavy_degree < 1 || k.pull_chunk < 1 || k.ss_chunk < 0 ||
+        (k.pull_balanced != 0 && k.pull_balanced != 1)) {
       rc = DAWN_ERR_FORMAT;
       break;
     }
@@ -183,8 +189,9 @@ int dawn_tune_save(const char* path) {
   if (!f) return DAWN_ERR_IO;
   std::lock_guard<std::mutex> l(g_tune_lock);
   for (auto& kv : g_tune_cache)
-    std::fprintf(f, "%s %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
-                 kv.second.heavy_degree, kv.second.pull_chunk, kv.second.ss_chunk);
+    std::fprintf(f, "%s %d %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
+                 kv.second.heavy_degree, kv.second.pull_chunk, kv.second.ss_chunk,
+                 kv.second.pull_balanced);
   std::fclose(f);
   return DAWN_OK;
 }
