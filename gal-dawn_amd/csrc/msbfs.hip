@@ -976,6 +976,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   DAWN_HIP(hipSetDevice(g->device));
   DAWN_TRY(ensure_ms(g));
   MsWorkspace* ws = g->ms;
+  DAWN_TRY(ws->distbuf.alloc((size_t)kLanes * g->n));
   hipStream_t s = g->stream;
   MsArgs a = make_args(g);
   SchedParams p = make_params(g, policy);
@@ -983,7 +984,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   const unsigned grid = ms_grid(g);
   const unsigned fgrid = ms_final_grid(g);
   unsigned long long* ls = ws->lane_stats.p;
-  // Same work as a sums-only traversal with statistics (no distance rows).
+  // Same work as a traversal that writes distance rows and statistics.
   MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 1};
   for (int i = 0; i < 8; i++) {
     kernel_ms[i] = 0;
@@ -1005,8 +1006,8 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   for (int32_t b0 = 0; b0 < S; b0 += kLanes, batch++) {
     const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
     stage_batch(ws, sources + b0, lanes);
-    ws->h_batch->dist = nullptr;
-    ws->h_batch->stride = 0;
+    ws->h_batch->dist = ws->distbuf.p;
+    ws->h_batch->stride = g->n;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     DAWN_TRY(timed(0, [&] { ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o); }));
     DAWN_TRY(timed(1, [&] { ms_seed_kernel<<<1, 64, 0, s>>>(a); }));
