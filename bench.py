@@ -32,11 +32,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def batch_bytes(n, m, levels):
-    """Algorithmic HBM bytes of one 64-source batch: offsets and columns
-    once, one 8-byte frontier word per directed edge, read+write of one
-    8-byte word per vertex per step, 64 int32 distances per vertex."""
-    return 4 * (n + 1) + 12 * m + 16 * n * levels + 256 * n
+def batch_bytes(n, m, levels, words=1):
+    """Algorithmic HBM bytes of one batch of 64*words sources: offsets and
+    columns once, one frontier record (8*words bytes) per directed edge,
+    read+write of one record per vertex per step, 64*words int32 distances
+    per vertex.  words=1 is the 64-source figure."""
+    return 4 * (n + 1) + 4 * m + 8 * words * m + 16 * words * n * levels + 256 * words * n
 
 
 def main():
@@ -94,11 +95,9 @@ def main():
         tune_s = time.time() - t0
         if args.tune_out:
             dawn.tune_save(args.tune_out)
-    with tempfile.NamedTemporaryFile("r", suffix=".txt") as tf:
-        dawn.tune_save(tf.name)
-        knobs = open(tf.name).read().split()
-    log(f"launch knobs (class dense_blocks heavy chunk ss_chunk balanced): {knobs or 'defaults'}"
-        f"{f' tuned in {tune_s:.1f}s' if args.retune else ''}")
+    knobs = g.knobs()
+    W = knobs["lane_words"]  # 64*W sources per batch (4096 sources: always the knob)
+    log(f"launch knobs: {knobs}{f' tuned in {tune_s:.1f}s' if args.retune else ''}")
 
     # Learned scheduler network (trained offline by gal-dawn_amd/train_sched.py).
     learned_loaded = False
@@ -178,14 +177,16 @@ def main():
     L = float(levels.mean())
     total_prof = sum(v["total_ms"] for v in kern.values())
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    # Algorithmic bytes per launch of each kernel: the bytes the step must move at least.
+    # Algorithmic bytes per launch of each kernel (batch of 64*W sources,
+    # W frontier words per vertex): the bytes the step must move at least.
+    step_edges = (4 * (n + 1) + 4 * m + 8 * W * m) / L + 16 * W * n  # batch edge traffic per step
     per_launch = {
-        "final": 8 * n * (L + 1) + 256 * n,           # seen + L level words in, 64 int32 out
-        "finalize": 24 * n,                           # next + seen in, level word out
-        "expand_pull": (4 * (n + 1) + 12 * m) / L + 16 * n,  # the batch's edge traffic per step
-        "expand_push": (4 * (n + 1) + 12 * m) / L + 16 * n,
-        "expand_bitmap": (4 * (n + 1) + 12 * m) / L + 16 * n,
-        "fill": 24 * n,
+        "final": 8 * W * n * (L + 1) + 256 * W * n,   # seen + L level records in, 64W int32 out
+        "finalize": 24 * W * n,                       # next + seen in, level record out
+        "expand_pull": step_edges,
+        "expand_push": step_edges,
+        "expand_bitmap": step_edges,
+        "fill": 24 * W * n,
     }
     dom_bytes = per_launch.get(dom, 0.0)
     dom_avg_s = kern[dom]["avg_ms"] / 1e3
@@ -201,9 +202,12 @@ def main():
         "avg_launch_ms": kern[dom]["avg_ms"],
         "bytes_per_launch": dom_bytes,
     }
-    b_batch = batch_bytes(n, m, L)
+    b_batch = batch_bytes(n, m, L, W)
     lt = results["learned"]["device_ms_per_step"] / 1e3
-    batch_frac = (b_batch * (args.sources / 64)) / lt / 1e9 / HBM_PEAK_GBS
+    batch_frac = (b_batch * (args.sources / (64 * W))) / lt / 1e9 / HBM_PEAK_GBS
+    # The 64-source figure of the same run: what the work would move in
+    # 64-lane batches (the bit-packing width the metric is defined on).
+    batch_frac_64 = (batch_bytes(n, m, L) * (args.sources / 64)) / lt / 1e9 / HBM_PEAK_GBS
     log(f"kernels: {kern}; levels/batch {L:.2f}; dominant {dom}")
 
     # Bit-exact spot check against the CPU oracle on sampled sources.
@@ -256,7 +260,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"RMAT a=.57 b=.19 c=.19 scale {args.scale} ef {args.edge_factor} symmetrized "
-                        f"dedup seed {args.seed}; unweighted MSSP of {args.sources} sources (64-bit packed)",
+                        f"dedup seed {args.seed}; unweighted MSSP of {args.sources} sources, {64 * W} "
+                        f"per batch bit-packed; every batch's int32 distance rows written to HBM",
             "n": n, "m": m, "sources": args.sources, "parallelism": "single GPU",
         },
         "sources_per_s": results["learned"]["sources_per_s"],
@@ -265,6 +270,8 @@ def main():
         "schedule": sched,
         "roofline": roofline,
         "batch_roofline_frac": batch_frac,
+        "batch_roofline_frac_64lane_bytes": batch_frac_64,
+        "lane_words": W,
         "kernels": kern,
         "levels_per_batch": L,
         "copy_gbps": copy_gbps,

@@ -131,6 +131,8 @@ def _sig(lib):
         "dawn_tune": (c.c_int, [P, c.c_int32]),
         "dawn_tune_load": (c.c_int, [c.c_char_p]),
         "dawn_tune_save": (c.c_int, [c.c_char_p]),
+        "dawn_graph_get_knobs": (c.c_int, [P, c.POINTER(c.c_int32)]),
+        "dawn_graph_set_knobs": (c.c_int, [P, c.POINTER(c.c_int32)]),
     }
     for name, (res, args) in defs.items():
         f = getattr(lib, name)
@@ -437,7 +439,28 @@ class Graph:
         _check(lib().dawn_mssp_profile(self._h, _ptr(src, ctypes.c_int32), len(src), _policy(policy),
                                        _ptr(ms, ctypes.c_double), _ptr(cnt, ctypes.c_int64),
                                        _ptr(levels, ctypes.c_int32)), "dawn_mssp_profile")
-        return ({k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PROF_NAMES)}, levels)
+        # One entry per batch of 64*W sources, -1 past the last batch.
+        return ({k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PROF_NAMES)},
+                levels[levels >= 0])
+
+    KNOB_NAMES = ("dense_blocks", "heavy_degree", "pull_chunk", "ss_chunk", "pull_variant",
+                  "lane_words")
+
+    def knobs(self) -> dict:
+        """The graph's launch knobs (dawn_graph_get_knobs)."""
+        k = np.zeros(len(self.KNOB_NAMES), np.int32)
+        _check(lib().dawn_graph_get_knobs(self._h, _ptr(k, ctypes.c_int32)), "dawn_graph_get_knobs")
+        return dict(zip(self.KNOB_NAMES, (int(x) for x in k)))
+
+    def set_knobs(self, **kw):
+        """Changes some launch knobs (time only, never results)."""
+        cur = self.knobs()
+        for name, val in kw.items():
+            if name not in cur:
+                raise KeyError(name)
+            cur[name] = int(val)
+        k = np.array([cur[n] for n in self.KNOB_NAMES], np.int32)
+        _check(lib().dawn_graph_set_knobs(self._h, _ptr(k, ctypes.c_int32)), "dawn_graph_set_knobs")
 
     def sched_record(self, sources, repeats: int = 3, max_rows: int = 4096):
         src = _i32(sources)

@@ -1,39 +1,48 @@
-// 64-source bit-packed unweighted DAWN (batched BOVM) on one HIP device.
+// Bit-packed unweighted DAWN over 64·W sources at once (batched BOVM) on one
+// HIP device, W = 1, 2 or 4 lane words per vertex.
 //
 // Replaces the reference's per-source loops DAWN::MSSP_GPU::run
 // (src/algorithm/gpu/mssp.cu:9-51), DAWN::APSP_GPU::run
 // (src/algorithm/gpu/apsp.cu:62-111) and DAWN::MSSP_CPU::run
-// (src/algorithm/cpu/mssp.cpp:9-32): one step advances 64 frontiers at
-// once, each vertex carrying one uint64 word with bit b = source lane b.
+// (src/algorithm/cpu/mssp.cpp:9-32): one step advances 64·W frontiers at
+// once; each vertex carries W uint64 words, bit b of word i = source lane
+// 64·i + b.  A vertex's W words are contiguous, so every random access of
+// a step (one frontier gather, one seen probe, one atomic) moves the lanes
+// of all W words in one cache line: the dense steps cost about the same
+// for 256 sources as for 64.
 //
-// Layout in HBM (per graph, reused across batches):
-//   seen   u64[npad]          lanes that have reached v
-//   next   u64[npad]          push/bitmap scatter target (kept all-zero
-//                             between steps by the finalize pass)
-//   hist   u64[H][npad]       level history: slot l%H holds the lanes that
-//                             reached v exactly at level l (= frontier of l)
-//   q, qb  int32[2][qcap]     frontier queue (level parity): one entry per
-//                             256 out-edges of a frontier vertex (vertex,
-//                             first edge), so push work per entry is bounded
-//   dist   int32[64][n]       output rows of the batch (caller or workspace)
+// Layout in HBM (per graph, reused across batches; v-major, W words each):
+//   seen   u64[npad][W]        lanes that have reached v
+//   next   u64[npad][W]        scatter target of the step (kept all-zero
+//                              between steps by the finalize pass)
+//   hist   u64[H][npad][W]     level history: slot l%H holds the lanes that
+//                              reached v exactly at level l (= frontier of l)
+//   fbits  u32[2][npad/32]     1 bit per vertex: in the frontier of a level
+//                              (any lane), slot = level parity
+//   q, qb  int32[qcap]         push queue: one entry per 256 out-edges of a
+//                              frontier vertex (vertex, first edge)
+//   dist   int32[64W][n]       output rows of the batch (caller or workspace)
 // Step modes (all produce the same next level in next[]):
-//   PULL    every vertex missing a lane ORs the previous level's words of its
-//           in-neighbours (CSC): light vertices one lane each with early exit
-//           once all missing lanes are found; heavy vertices' in-edge lists
-//           in 256-edge chunks, one wave per chunk, merged by atomicOr.
+//   PULL    every vertex missing a live lane ORs the previous level's words
+//           of its in-neighbours that are in the frontier bitmap (CSC):
+//           light vertices one lane each with early exit once all missing
+//           lanes are found (or edge-balanced over the workgroup); heavy
+//           vertices' in-edge lists in chunks, one wave per chunk, merged by
+//           atomicOr.
 //   PUSH    queue entries, workgroup-balanced expansion (LDS degree scan),
-//           64-bit atomicOr of (frontier & ~seen[v]) into next[v].
-//   BITMAP  dense scan of the previous level slot, same scatter as PUSH.
+//           atomicOr of (frontier & ~seen[v]) into next[v], word by word.
+//   BITMAP  dense scan of the frontier bitmap, same scatter as PUSH.
 //   Every step ends with a finalize pass that folds next into the level
-//   slot, seen, the queue and the level counters.
+//   slot, seen, the frontier bitmap and the level counters.
 // After the last level one pass turns the history into int32 distances
-// (64 coalesced row stores per wave) and per-lane sums/counts (64x64 bit
-// transposes + popcount), so no distance is written twice.
+// (16-byte row stores) and per-lane sums/counts (64x64 bit transposes +
+// popcount), so no distance is written twice.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -41,18 +50,19 @@
 
 namespace dawn {
 
-constexpr int kLanes = 64;
+constexpr int kMaxWords = 4;
+constexpr int kMaxLanes = 64 * kMaxWords;
 constexpr int kHist = 32;        // level slots
 constexpr int kChunk = 24;       // steps per captured chunk (< kHist)
 constexpr int kSlices = 6;       // bits of a relative level (2^6 > kHist)
 
 struct MsBatch {
   int32_t nlanes;
-  int32_t lo;          // first level not yet flushed to dist
-  uint64_t lanemask;   // valid lanes
-  int32_t* dist;       // output rows: lane b at dist + b * stride
+  int32_t lo;                      // first level not yet flushed to dist
+  uint64_t lanemask[kMaxWords];    // valid lanes per word
+  int32_t* dist;                   // output rows: lane b at dist + b * stride
   int64_t stride;
-  int32_t src[kLanes];
+  int32_t src[kMaxLanes];
 };
 
 struct MsArgs {
@@ -67,10 +77,10 @@ struct MsArgs {
   uint64_t* seen;
   uint64_t* next;
   uint64_t* hist;
-  uint32_t* fbits;     // [2][fwords] 1 bit per vertex: in the frontier of a
-  int64_t fwords;      // level (any lane), slot = level parity
-  int32_t* q;          // [2][qcap] frontier entries: vertex
-  int32_t* qb;         // [2][qcap] first out-edge of the entry's chunk
+  uint32_t* fbits;     // [2][fwords]
+  int64_t fwords;
+  int32_t* q;          // [qcap] frontier entries: vertex
+  int32_t* qb;         // [qcap] first out-edge of the entry's chunk
   int64_t qcap;
   const int32_t* hc_v;    // heavy in-edge chunks (pull)
   const int32_t* hc_beg;
@@ -81,7 +91,7 @@ struct MsArgs {
   const int32_t* oc_end;
   int64_t num_oc;
   int32_t heavy;          // degree from which a vertex's edges are chunked
-  int32_t balanced;       // light pull edge-balanced over the workgroup
+  int32_t variant;        // light pull variant (TuneKnobs::pull_variant)
   int32_t deg_bits;       // bits of the largest out-degree
   LevelCounters* ring;
   SchedState* st;
@@ -97,6 +107,7 @@ struct MsOut {
 
 struct MsWorkspace {
   int32_t n = 0;
+  int32_t words = 0;   // W the buffers are laid out for
   int64_t npad = 0;
   DevBuf<uint64_t> seen, next, hist, seen_bak;
   DevBuf<uint32_t> fbits;
@@ -107,17 +118,18 @@ struct MsWorkspace {
   DevBuf<MsBatch> batch;
   DevBuf<float> trace_feat;
   DevBuf<int32_t> trace_mode;
-  DevBuf<unsigned long long> lane_stats;  // 3 * 64
+  DevBuf<unsigned long long> lane_stats;  // 3 * kMaxLanes
   SchedState* h_state = nullptr;
   MsBatch* h_batch = nullptr;
-  std::map<std::pair<long long, const void*>, hipGraphExec_t> graphs;
+  // Captured batch graphs keyed by (W, policy, want_edges, scheduler net).
+  std::map<std::tuple<int, int, int, const void*>, hipGraphExec_t> graphs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, e_a = nullptr, e_b = nullptr;
 };
 
 void ss_invalidate_graphs(SsWorkspace* ws);
 
-// Captured step graphs bake grid sizes; the autotuner drops them when it
-// changes the knobs.
+// Captured step graphs bake grid sizes and buffer addresses; the autotuner
+// and workspace re-layouts drop them.
 void invalidate_graphs(dawn_graph* g) {
   if (g->ms) {
     for (auto& kv : g->ms->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -152,26 +164,52 @@ __device__ __forceinline__ int owner_of(const int* off, int e) {
   return lo;
 }
 
+// The W words of one vertex: one 8-, 16- or 32-byte load.
+template <int W>
+__device__ __forceinline__ void load_words(const uint64_t* p, uint64_t (&x)[W]) {
+  if constexpr (W == 1) {
+    x[0] = p[0];
+  } else {
+    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+#pragma unroll
+    for (int i = 0; i < W / 2; i++) {
+      const ulonglong2 t = q[i];
+      x[2 * i] = t.x;
+      x[2 * i + 1] = t.y;
+    }
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void or_words(uint64_t* p, const uint64_t (&x)[W]) {
+#pragma unroll
+  for (int i = 0; i < W; i++)
+    if (x[i]) atomicOr((unsigned long long*)&p[i], (unsigned long long)x[i]);
+}
+
 // Per-thread counters of a level, reduced over the workgroup and flushed
 // with one set of atomics per workgroup (same-address atomics from every
 // wave serialise in L2).  flush() must be called by every thread.
+template <int W>
 struct MsAcc {
   unsigned long long fv = 0, fe = 0, scout = 0, lanes = 0, uv = 0, ue = 0;
-  uint64_t alive = 0;
+  uint64_t alive[W] = {};
   __device__ void flush(LevelCounters* c) {
-    __shared__ unsigned long long s_v[kBlock / kWave][7];
+    constexpr int R = 6 + W;
+    __shared__ unsigned long long s_v[kBlock / kWave][R];
     const int w = threadIdx.x / kWave;
-    const unsigned long long r[7] = {wave_sum_u64(fv),    wave_sum_u64(fe), wave_sum_u64(scout),
-                                     wave_sum_u64(lanes), wave_sum_u64(uv), wave_sum_u64(ue),
-                                     wave_or_u64(alive)};
+    unsigned long long r[R] = {wave_sum_u64(fv),    wave_sum_u64(fe), wave_sum_u64(scout),
+                               wave_sum_u64(lanes), wave_sum_u64(uv), wave_sum_u64(ue)};
+#pragma unroll
+    for (int i = 0; i < W; i++) r[6 + i] = wave_or_u64(alive[i]);
     if (lane_id() == 0)
-      for (int i = 0; i < 7; i++) s_v[w][i] = r[i];
+      for (int i = 0; i < R; i++) s_v[w][i] = r[i];
     __syncthreads();
     if (threadIdx.x == 0) {
-      unsigned long long t[7] = {0, 0, 0, 0, 0, 0, 0};
+      unsigned long long t[R] = {};
       for (int j = 0; j < kBlock / kWave; j++) {
         for (int i = 0; i < 6; i++) t[i] += s_v[j][i];
-        t[6] |= s_v[j][6];
+        for (int i = 6; i < R; i++) t[i] |= s_v[j][i];
       }
       if (t[0]) {
         atomicAdd(&c->fv, t[0]);
@@ -183,58 +221,70 @@ struct MsAcc {
         atomicAdd(&c->uv, t[4]);
         atomicAdd(&c->ue, t[5]);
       }
-      if (t[6]) atomicOr((unsigned long long*)&c->alive, t[6]);
+      for (int i = 0; i < W; i++)
+        if (t[6 + i]) atomicOr((unsigned long long*)&c->alive[i], t[6 + i]);
     }
   }
 };
 
 // ---------------------------------------------------------------- kernels ---
 
+template <int W>
 __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
-  const uint64_t inval = ~a.batch->lanemask;
+  const MsBatch& b = *a.batch;
   const int64_t gs = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.npad; v += gs) {
-    a.seen[v] = v < a.n ? inval : ~0ull;
-    a.hist[v] = 0ull;  // level-0 slot
-    a.next[v] = 0ull;
+  const int64_t words = a.npad * W;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < words; x += gs) {
+    const int64_t v = x / W;
+    a.seen[x] = v < a.n ? ~b.lanemask[x % W] : ~0ull;
+    a.hist[x] = 0ull;  // level-0 slot
+    a.next[x] = 0ull;
   }
   for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.fwords; w += gs)
     a.fbits[w] = 0u;  // level-0 frontier bitmap
   if (blockIdx.x == 0 && threadIdx.x < kRing) a.ring[threadIdx.x] = LevelCounters{};
-  if (blockIdx.x == 0 && threadIdx.x < kLanes) {
-    o.sum[threadIdx.x] = 0;
-    o.reached[threadIdx.x] = 0;
-    o.edges[threadIdx.x] = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 64 * W; t += gs) {
+    o.sum[t] = 0;
+    o.reached[t] = 0;
+    o.edges[t] = 0;
   }
 }
 
 // Level 0: the batch's sources (duplicates share one frontier vertex).
+template <int W>
 __global__ void ms_seed_kernel(MsArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const MsBatch& b = *a.batch;
   LevelCounters c{};
-  for (int i = 0; i < b.nlanes; i++) {
-    const int32_t v = b.src[i];
-    const uint64_t bit = 1ull << i;
-    if (a.hist[v] == 0) {
+  for (int l = 0; l < b.nlanes; l++) {
+    const int32_t v = b.src[l];
+    const int i = l >> 6;
+    const uint64_t bit = 1ull << (l & 63);
+    uint64_t any = 0;
+    for (int k = 0; k < W; k++) any |= a.hist[(int64_t)v * W + k];
+    if (!any) {
       const int d = out_deg(a, v);
       a.fbits[v >> 5] |= 1u << (v & 31);
       c.fv += 1;
       c.fe += (unsigned)d;
       c.scout += (unsigned)d;
     }
-    a.hist[v] |= bit;
-    a.seen[v] |= bit;
+    a.hist[(int64_t)v * W + i] |= bit;
+    a.seen[(int64_t)v * W + i] |= bit;
     c.flanes += 1;
-    c.alive |= bit;
+    c.alive[i] |= bit;
   }
-  // Vertices still missing a lane after level 0, and their in-degree sum.
+  // Vertices still missing a lane after level 0, and their in-degree sum:
+  // a source vertex counts once (at its first lane) if every lane has it.
   c.uv = (unsigned long long)a.n;
   c.ue = (unsigned long long)a.m;
-  for (int i = 0; i < b.nlanes; i++) {
-    const int32_t v = b.src[i];
-    const bool first = (a.hist[v] & ((1ull << i) - 1)) == 0;  // lowest lane of v
-    if (first && a.seen[v] == ~0ull) {
+  for (int l = 0; l < b.nlanes; l++) {
+    const int32_t v = b.src[l];
+    bool first = true;
+    for (int k = 0; k < l; k++) first = first && b.src[k] != v;
+    bool full = true;
+    for (int k = 0; k < W; k++) full = full && a.seen[(int64_t)v * W + k] == ~0ull;
+    if (first && full) {
       c.uv -= 1;
       c.ue -= (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
     }
@@ -253,15 +303,81 @@ __global__ void ms_decide_kernel(MsArgs a, SchedParams p, SchedTrace tr) {
   sched_decide(a.st, a.ring, p, tr);
 }
 
+// Light vertices of a pull step (in-degree < a.heavy), one lane each: U
+// column reads, then U bitmap probes, then the gathers they admit, between
+// early-exit checks (all live missing lanes found).  NT: column reads as
+// streaming loads, so the in-edge lists do not displace the frontier words
+// and bitmap in the caches.  The lane owns next[v].
+template <int W, int U, bool NT>
+__device__ __forceinline__ void pull_light_lanes(const MsArgs& a, const uint64_t* prev,
+                                                 const uint32_t* fb, const uint64_t (&alive)[W]) {
+  const int lane = lane_id();
+  const int64_t wstride = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+       base += wstride) {
+    const int32_t v = (int32_t)(base + lane);
+    if (v >= a.n) continue;
+    uint64_t miss[W], sv[W];
+    load_words<W>(a.seen + (int64_t)v * W, sv);
+    uint64_t any = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+      miss[i] = ~sv[i] & alive[i];
+      any |= miss[i];
+    }
+    if (!any) continue;
+    const int beg = a.in_row_ptr[v];
+    const int deg = a.in_row_ptr[v + 1] - beg;
+    if (deg >= a.heavy || deg == 0) continue;
+    const int32_t* nb = a.in_col + beg;
+    uint64_t acc[W] = {};
+    auto complete = [&]() {
+      bool c = true;
+#pragma unroll
+      for (int i = 0; i < W; i++) c = c && (acc[i] & miss[i]) == miss[i];
+      return c;
+    };
+    auto gather = [&](int32_t u) {
+      uint64_t f[W];
+      load_words<W>(prev + (int64_t)u * W, f);
+#pragma unroll
+      for (int i = 0; i < W; i++) acc[i] |= f[i];
+    };
+    int j = 0;
+    for (; j + U <= deg; j += U) {
+      int32_t u[U];
+      uint32_t w[U];
+#pragma unroll
+      for (int t = 0; t < U; t++) u[t] = NT ? __builtin_nontemporal_load(nb + j + t) : nb[j + t];
+#pragma unroll
+      for (int t = 0; t < U; t++) w[t] = fb[u[t] >> 5];
+#pragma unroll
+      for (int t = 0; t < U; t++)
+        if ((w[t] >> (u[t] & 31)) & 1u) gather(u[t]);
+      if (complete()) break;
+    }
+    if (!complete())
+      for (; j < deg; j++) {
+        const int32_t u = NT ? __builtin_nontemporal_load(nb + j) : nb[j];
+        if ((fb[u >> 5] >> (u & 31)) & 1u) gather(u);
+      }
+    uint64_t* nx = a.next + (int64_t)v * W;
+#pragma unroll
+    for (int i = 0; i < W; i++)
+      if (acc[i] & miss[i]) nx[i] = acc[i] & miss[i];
+  }
+}
+
+template <int W>
 __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   __shared__ int s_off[kBlock];
   __shared__ int s_beg[kBlock];
-  __shared__ uint64_t s_f[kBlock];
+  __shared__ uint64_t s_f[kBlock * W];
   __shared__ int s_scan[kBlock / kWave];
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
-  const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad;
+  const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad * W;
   const int lane = lane_id();
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
@@ -273,24 +389,28 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
       const int64_t i = c0 + threadIdx.x;
       int beg = 0, deg = 0;
-      uint64_t f = 0;
+      uint64_t f[W] = {};
       if (i < qn) {
         const int32_t u = q_in[i];
         beg = qb_in[i];
         deg = min(a.row_ptr[u + 1] - beg, kPushChunk);
-        f = prev[u];
+        load_words<W>(prev + (int64_t)u * W, f);
       }
       int total;
       const int off = block_exclusive_scan(deg, s_scan, &total);
       s_off[threadIdx.x] = off;
       s_beg[threadIdx.x] = beg;
-      s_f[threadIdx.x] = f;
+#pragma unroll
+      for (int k = 0; k < W; k++) s_f[threadIdx.x * W + k] = f[k];
       __syncthreads();
       for (int e = threadIdx.x; e < total; e += kBlock) {
         const int j = owner_of(s_off, e);
         const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
-        const uint64_t nv = s_f[j] & ~a.seen[v];
-        if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
+        uint64_t sv[W], nv[W];
+        load_words<W>(a.seen + (int64_t)v * W, sv);
+#pragma unroll
+        for (int k = 0; k < W; k++) nv[k] = s_f[j * W + k] & ~sv[k];
+        or_words<W>(a.next + (int64_t)v * W, nv);
       }
       __syncthreads();
     }
@@ -303,48 +423,71 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
          base += wstride) {
       const int32_t u = (int32_t)(base + lane);
       if (u >= a.n || !((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
-      const uint64_t f = prev[u];
       const int beg = a.row_ptr[u];
       const int deg = a.row_ptr[u + 1] - beg;
       if (deg >= a.heavy) continue;
+      uint64_t f[W];
+      load_words<W>(prev + (int64_t)u * W, f);
       for (int j = 0; j < deg; j++) {
         const int32_t v = a.col[beg + j];
-        const uint64_t nv = f & ~a.seen[v];
-        if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
+        uint64_t sv[W], nv[W];
+        load_words<W>(a.seen + (int64_t)v * W, sv);
+#pragma unroll
+        for (int k = 0; k < W; k++) nv[k] = f[k] & ~sv[k];
+        or_words<W>(a.next + (int64_t)v * W, nv);
       }
     }
     const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t c = wave; c < a.num_oc; c += nwaves) {
       const int32_t u = a.oc_v[c];
-      const uint64_t f = prev[u];
-      if (!f) continue;
+      uint64_t f[W];
+      load_words<W>(prev + (int64_t)u * W, f);
+      uint64_t any = 0;
+#pragma unroll
+      for (int k = 0; k < W; k++) any |= f[k];
+      if (!any) continue;
       const int e = a.oc_end[c];
       for (int k = a.oc_beg[c] + lane; k < e; k += kWave) {
         const int32_t v = a.col[k];
-        const uint64_t nv = f & ~a.seen[v];
-        if (nv) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)nv);
+        uint64_t sv[W], nv[W];
+        load_words<W>(a.seen + (int64_t)v * W, sv);
+#pragma unroll
+        for (int t = 0; t < W; t++) nv[t] = f[t] & ~sv[t];
+        or_words<W>(a.next + (int64_t)v * W, nv);
       }
     }
   } else {  // DAWN_MODE_PULL
     // Only lanes alive in the previous level can arrive now: a vertex looks
     // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
-    const uint64_t alive = a.ring[(lv - 1) % kRing].alive;
+    uint64_t alive[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) alive[k] = a.ring[(lv - 1) % kRing].alive[k];
     // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
-    // a 64-bit word is gathered only for in-neighbours that are in it.
+    // frontier words are gathered only for in-neighbours that are in it.
     const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
-    if (a.balanced) {
+    if (a.variant == 1) {
       // Light vertices, edge-balanced: the workgroup's 256 vertices' in-edge
       // lists are concatenated (degree scan) and read one edge per thread,
-      // coalesced; hits are OR-ed into the owner's LDS word.  Gathers stop
+      // coalesced; hits are OR-ed into the owner's LDS words.  Gathers stop
       // for an owner once it has found all its missing lanes.
-      __shared__ unsigned long long s_acc[kBlock];
+      __shared__ unsigned long long s_acc[kBlock * W];
       for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += wstride) {
         const int32_t v = (int32_t)(b0 + threadIdx.x);
-        const uint64_t miss = v < a.n ? (~a.seen[v] & alive) : 0ull;
+        uint64_t miss[W] = {};
+        uint64_t any = 0;
+        if (v < a.n) {
+          uint64_t sv[W];
+          load_words<W>(a.seen + (int64_t)v * W, sv);
+#pragma unroll
+          for (int k = 0; k < W; k++) {
+            miss[k] = ~sv[k] & alive[k];
+            any |= miss[k];
+          }
+        }
         int beg = 0, deg = 0;
-        if (miss) {
+        if (any) {
           beg = a.in_row_ptr[v];
           const int d = a.in_row_ptr[v + 1] - beg;
           deg = d < a.heavy ? d : 0;
@@ -353,51 +496,46 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
         const int off = block_exclusive_scan(deg, s_scan, &total);
         s_off[threadIdx.x] = off;
         s_beg[threadIdx.x] = beg;
-        s_f[threadIdx.x] = miss;
-        s_acc[threadIdx.x] = 0ull;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+          s_f[threadIdx.x * W + k] = miss[k];
+          s_acc[threadIdx.x * W + k] = 0ull;
+        }
         __syncthreads();
         for (int e = threadIdx.x; e < total; e += kBlock) {
           const int j = owner_of(s_off, e);
           const int32_t u = a.in_col[s_beg[j] + (e - s_off[j])];
           if ((fb[u >> 5] >> (u & 31)) & 1u) {
-            const uint64_t mj = s_f[j];
-            if ((__atomic_load_n(&s_acc[j], __ATOMIC_RELAXED) & mj) != mj) {
-              const uint64_t hit = prev[u] & mj;
-              if (hit) atomicOr(&s_acc[j], (unsigned long long)hit);
+            bool done = true;
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+              const uint64_t mj = s_f[j * W + k];
+              done = done && (__atomic_load_n(&s_acc[j * W + k], __ATOMIC_RELAXED) & mj) == mj;
+            }
+            if (!done) {
+              uint64_t f[W];
+              load_words<W>(prev + (int64_t)u * W, f);
+#pragma unroll
+              for (int k = 0; k < W; k++) {
+                const uint64_t hit = f[k] & s_f[j * W + k];
+                if (hit) atomicOr(&s_acc[j * W + k], (unsigned long long)hit);
+              }
             }
           }
         }
         __syncthreads();
-        if (deg && s_acc[threadIdx.x]) a.next[v] = s_acc[threadIdx.x];
-      }
-    } else
-    // Light vertices (in-degree < a.heavy): one lane each, four gathers
-    // in flight between early-exit checks; the lane owns next[v].
-    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
-         base += wstride) {
-      const int32_t v = (int32_t)(base + lane);
-      const uint64_t miss = v < a.n ? (~a.seen[v] & alive) : 0ull;
-      if (!miss) continue;
-      const int beg = a.in_row_ptr[v];
-      const int deg = a.in_row_ptr[v + 1] - beg;
-      if (deg >= a.heavy || deg == 0) continue;
-      const int32_t* nb = a.in_col + beg;
-      uint64_t acc_w = 0;
-      int j = 0;
-      for (; j + 4 <= deg; j += 4) {
-        const int32_t u0 = nb[j], u1 = nb[j + 1], u2 = nb[j + 2], u3 = nb[j + 3];
-        if ((fb[u0 >> 5] >> (u0 & 31)) & 1u) acc_w |= prev[u0];
-        if ((fb[u1 >> 5] >> (u1 & 31)) & 1u) acc_w |= prev[u1];
-        if ((fb[u2 >> 5] >> (u2 & 31)) & 1u) acc_w |= prev[u2];
-        if ((fb[u3 >> 5] >> (u3 & 31)) & 1u) acc_w |= prev[u3];
-        if ((acc_w & miss) == miss) break;
-      }
-      if ((acc_w & miss) != miss)
-        for (; j < deg; j++) {
-          const int32_t u = nb[j];
-          if ((fb[u >> 5] >> (u & 31)) & 1u) acc_w |= prev[u];
+        if (deg) {
+#pragma unroll
+          for (int k = 0; k < W; k++)
+            if (s_acc[threadIdx.x * W + k]) a.next[(int64_t)v * W + k] = s_acc[threadIdx.x * W + k];
         }
-      if (acc_w & miss) a.next[v] = acc_w & miss;
+      }
+    } else if (a.variant == 0) {
+      pull_light_lanes<W, 4, false>(a, prev, fb, alive);
+    } else if (a.variant == 2) {
+      pull_light_lanes<W, 8, false>(a, prev, fb, alive);
+    } else {
+      pull_light_lanes<W, 8, true>(a, prev, fb, alive);
     }
     // Heavy vertices: their in-edge lists are cut into chunks of at most
     // knobs.pull_chunk edges, one wave per chunk, merged with atomicOr.
@@ -405,61 +543,111 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t c = wave; c < a.num_hc; c += nwaves) {
       const int32_t v = a.hc_v[c];
-      const uint64_t miss = ~a.seen[v] & alive;
-      if (!miss) continue;
-      if ((a.next[v] & miss) == miss) continue;  // other chunks already found them all
+      uint64_t sv[W], nx[W], miss[W];
+      load_words<W>(a.seen + (int64_t)v * W, sv);
+      load_words<W>(a.next + (int64_t)v * W, nx);
+      bool found = true;
+      uint64_t any = 0;
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        miss[k] = ~sv[k] & alive[k];
+        any |= miss[k];
+        found = found && (nx[k] & miss[k]) == miss[k];  // other chunks found them all
+      }
+      if (!any || found) continue;
       const int he = a.hc_end[c];
-      uint64_t part = 0;
+      uint64_t part[W] = {};
       for (int k = a.hc_beg[c] + lane; k < he; k += kWave) {
         const int32_t u = a.in_col[k];
-        if ((fb[u >> 5] >> (u & 31)) & 1u) part |= prev[u];
+        if ((fb[u >> 5] >> (u & 31)) & 1u) {
+          uint64_t f[W];
+          load_words<W>(prev + (int64_t)u * W, f);
+#pragma unroll
+          for (int t = 0; t < W; t++) part[t] |= f[t];
+        }
       }
-      part = wave_or_u64(part) & miss;
-      if (lane == 0 && part) atomicOr((unsigned long long*)&a.next[v], (unsigned long long)part);
+#pragma unroll
+      for (int t = 0; t < W; t++) part[t] = wave_or_u64(part[t]) & miss[t];
+      if (lane == 0) or_words<W>(a.next + (int64_t)v * W, part);
     }
   }
 }
 
-// Folds next[] into the level slot, seen, the queue and the counters.
+// Folds next[] into the level slot, seen, the frontier bitmap and the
+// counters.  Each thread takes four consecutive words (4/W vertices) with
+// two 16-byte loads per array; npad is a multiple of 64, so a thread's
+// vertices never straddle it.
+template <int W>
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
+  constexpr int V = 4 / W;     // vertices per thread
+  constexpr int LPW = 32 / V;  // lanes per 32-bit bitmap word
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
-  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad;
+  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
   uint32_t* fb = a.fbits + (int64_t)(lv & 1) * a.fwords;
   LevelCounters* c_out = a.ring + (lv % kRing);
-  const uint64_t alive_prev = a.ring[(lv - 1) % kRing].alive;
-  MsAcc acc;
-  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.npad; b0 += (int64_t)gridDim.x * kBlock) {
-    const int32_t v = (int32_t)(b0 + threadIdx.x);
-    uint64_t nw = 0;
-    if (v < a.n) {
-      const uint64_t nx = a.next[v];
-      const uint64_t sv = a.seen[v];
-      if (nx) a.next[v] = 0ull;
-      nw = nx & ~sv;
-      cur[v] = nw;
-      if (nw) {
-        a.seen[v] = sv | nw;
-        const int d = out_deg(a, v);
-        acc.fv += 1;
-        acc.fe += (unsigned)d;
-        acc.scout += (unsigned)(d > 0 ? d : 1);
-        acc.lanes += (unsigned)__popcll(nw);
-        acc.alive |= nw;
-      }
-      if ((~(sv | nw) & alive_prev) != 0) {  // still misses a live lane
-        acc.uv += 1;
-        acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+  uint64_t alive_prev[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) alive_prev[k] = a.ring[(lv - 1) % kRing].alive[k];
+  const int lane = lane_id();
+  MsAcc<W> acc;
+  constexpr int64_t kTile = (int64_t)kBlock * V;
+  for (int64_t b0 = (int64_t)blockIdx.x * kTile; b0 < a.npad; b0 += (int64_t)gridDim.x * kTile) {
+    const int64_t v0 = b0 + (int64_t)threadIdx.x * V;
+    unsigned nib = 0;
+    if (v0 < a.npad) {
+      const int64_t w0 = v0 * W;
+      ulonglong2* np2 = reinterpret_cast<ulonglong2*>(a.next + w0);
+      ulonglong2* sp2 = reinterpret_cast<ulonglong2*>(a.seen + w0);
+      ulonglong2* cp2 = reinterpret_cast<ulonglong2*>(cur + w0);
+      const ulonglong2 n01 = np2[0], n23 = np2[1];
+      const ulonglong2 s01 = sp2[0], s23 = sp2[1];
+      const uint64_t nx[4] = {n01.x, n01.y, n23.x, n23.y};
+      const uint64_t sv[4] = {s01.x, s01.y, s23.x, s23.y};
+      uint64_t nw[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) nw[k] = nx[k] & ~sv[k];
+      if (nx[0] | nx[1]) np2[0] = make_ulonglong2(0ull, 0ull);
+      if (nx[2] | nx[3]) np2[1] = make_ulonglong2(0ull, 0ull);
+      cp2[0] = make_ulonglong2(nw[0], nw[1]);
+      cp2[1] = make_ulonglong2(nw[2], nw[3]);
+      if (nw[0] | nw[1]) sp2[0] = make_ulonglong2(sv[0] | nw[0], sv[1] | nw[1]);
+      if (nw[2] | nw[3]) sp2[1] = make_ulonglong2(sv[2] | nw[2], sv[3] | nw[3]);
+      // Padding vertices (v >= n) have seen = ~0 and next = 0: never new,
+      // never unvisited.
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const int32_t v = (int32_t)(v0 + j);
+        uint64_t any = 0, unv = 0;
+        unsigned pc = 0;
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+          const uint64_t x = nw[j * W + i];
+          any |= x;
+          pc += (unsigned)__popcll(x);
+          acc.alive[i] |= x;
+          unv |= ~(sv[j * W + i] | x) & alive_prev[i];
+        }
+        if (any) {
+          nib |= 1u << j;
+          const int d = out_deg(a, v);
+          acc.fv += 1;
+          acc.fe += (unsigned)d;
+          acc.scout += (unsigned)(d > 0 ? d : 1);
+          acc.lanes += pc;
+        }
+        if (unv) {  // still misses a live lane
+          acc.uv += 1;
+          acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+        }
       }
     }
-    // Frontier bitmap of the level: each wave owns its 64 vertices' bits.
-    const uint64_t fm = __ballot(nw != 0);
-    if (lane_id() == 0) {
-      const int64_t w = (b0 + (threadIdx.x & ~(kWave - 1))) >> 5;
-      fb[w] = (uint32_t)fm;
-      fb[w + 1] = (uint32_t)(fm >> 32);
-    }
+    // Frontier bitmap of the level: LPW lanes (32 vertices) per word.
+    unsigned x = nib << (V * (lane % LPW));
+#pragma unroll
+    for (int o = 1; o < LPW; o <<= 1) x |= __shfl_xor(x, o);
+    if ((lane % LPW) == 0 && v0 < a.npad) fb[v0 >> 5] = x;
   }
   acc.flush(c_out);
 }
@@ -497,9 +685,14 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
 // Turns levels [lo, hi) of the history into distances.  from_state: lo and
 // hi come from the batch / scheduler state and the pass runs only once the
 // traversal is done (then also writes INF for lanes that never arrived).
+// Wave w of a workgroup handles word w % W of a 256-vertex tile (4 vertices
+// per lane), so the 64 rows of one word are written with 16-byte stores,
+// 1 KB per wave per row, and the W waves of a tile share its cache lines.
+template <int W>
 __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int from_state,
                                                           int lo_arg, int hi_arg, int final_arg) {
-  __shared__ unsigned long long s_sum[kLanes], s_reach[kLanes], s_edges[kLanes];
+  constexpr int TPB = 4 / W;  // vertex tiles per workgroup
+  __shared__ unsigned long long s_sum[64 * W], s_reach[64 * W], s_edges[64 * W];
   int lo = lo_arg, hi = hi_arg, fin = final_arg;
   if (from_state) {
     const SchedState st = *a.st;
@@ -511,7 +704,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   const int nl = a.batch->nlanes;
   int32_t* const dist = a.batch->dist;
   const int64_t stride = a.batch->stride;
-  if (threadIdx.x < kLanes) {
+  if (threadIdx.x < 64 * W) {
     s_sum[threadIdx.x] = 0;
     s_reach[threadIdx.x] = 0;
     s_edges[threadIdx.x] = 0;
@@ -522,13 +715,15 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   int K = 0;
   while ((1 << K) < hi - lo) K++;
   const int lane = lane_id();
-  // Each lane owns 4 consecutive vertices (a wave 256), so a row store is
-  // 16 bytes per lane, 1 KB per wave.  With one flush of every level
-  // (lo == 0, final) every (lane, vertex) is written: level or INF.
+  const int wv = threadIdx.x / kWave;
+  const int wi = wv % W;     // lane word of this wave
+  const int tile = wv / W;   // tile of this wave within the workgroup
+  const int rows = min(64, nl - 64 * wi);  // lanes of this word in the batch
+  // With one flush of every level (lo == 0, final) every (lane, vertex) is
+  // written: level or INF.
   const bool vec_ok = dist && (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0) && lo == 0 && fin;
-  const int64_t wstride = (int64_t)gridDim.x * kBlock * 4;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock * 4 + (threadIdx.x & ~(kWave - 1)) * 4; base < a.n;
-       base += wstride) {
+  const int64_t wstride = (int64_t)gridDim.x * TPB * 256;
+  for (int64_t base = ((int64_t)blockIdx.x * TPB + tile) * 256; base < a.n; base += wstride) {
     const int64_t v0 = base + 4 * lane;
     uint64_t F[4], S[4][kSlices], sv[4];
 #pragma unroll
@@ -539,13 +734,25 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       for (int k = 0; k < kSlices; k++) S[j][k] = 0;
     }
     if (v0 + 3 < a.n) {
-      const ulonglong2* sp = reinterpret_cast<const ulonglong2*>(a.seen + v0);
-      const ulonglong2 s01 = sp[0], s23 = sp[1];
-      sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
+      if constexpr (W == 1) {
+        const ulonglong2* sp = reinterpret_cast<const ulonglong2*>(a.seen + v0);
+        const ulonglong2 s01 = sp[0], s23 = sp[1];
+        sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) sv[j] = a.seen[(v0 + j) * W + wi];
+      }
       for (int l = lo; l < hi; l++) {
-        const ulonglong2* hp = reinterpret_cast<const ulonglong2*>(a.hist + (int64_t)(l % a.H) * a.npad + v0);
-        const ulonglong2 h01 = hp[0], h23 = hp[1];
-        const uint64_t w[4] = {h01.x, h01.y, h23.x, h23.y};
+        const uint64_t* hb = a.hist + (int64_t)(l % a.H) * a.npad * W;
+        uint64_t w[4];
+        if constexpr (W == 1) {
+          const ulonglong2* hp = reinterpret_cast<const ulonglong2*>(hb + v0);
+          const ulonglong2 h01 = hp[0], h23 = hp[1];
+          w[0] = h01.x; w[1] = h01.y; w[2] = h23.x; w[3] = h23.y;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; j++) w[j] = hb[(v0 + j) * W + wi];
+        }
         const int r = l - lo;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -559,9 +766,9 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       for (int j = 0; j < 4; j++) {
         const int64_t v = v0 + j;
         if (v >= a.n) continue;
-        sv[j] = a.seen[v];
+        sv[j] = a.seen[v * W + wi];
         for (int l = lo; l < hi; l++) {
-          const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad + v];
+          const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad * W + v * W + wi];
           const int r = l - lo;
           F[j] |= w;
 #pragma unroll
@@ -572,7 +779,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
     }
     if (dist && v0 < a.n) {
       const bool full = vec_ok && v0 + 3 < a.n;
-      for (int b = 0; b < nl; b++) {
+      for (int b = 0; b < rows; b++) {
         int32_t d[4];
         bool wr[4];
 #pragma unroll
@@ -588,7 +795,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
           }
           d[j] = x;
         }
-        int32_t* row = dist + (int64_t)b * stride + v0;
+        int32_t* row = dist + (int64_t)(64 * wi + b) * stride + v0;
         if (full) {
           *reinterpret_cast<int4*>(row) = make_int4(d[0], d[1], d[2], d[3]);
         } else {
@@ -631,42 +838,59 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         }
       }
     }
-    if (sum) atomicAdd(&s_sum[lane], sum);
-    if (reach) atomicAdd(&s_reach[lane], reach);
-    if (edges) atomicAdd(&s_edges[lane], edges);
+    const int gl = 64 * wi + lane;
+    if (sum) atomicAdd(&s_sum[gl], sum);
+    if (reach) atomicAdd(&s_reach[gl], reach);
+    if (edges) atomicAdd(&s_edges[gl], edges);
   }
   __syncthreads();
-  if (threadIdx.x < nl) {
-    const int b = threadIdx.x;
+  for (int b = threadIdx.x; b < nl; b += kBlock) {
     if (s_sum[b]) atomicAdd(&o.sum[b], s_sum[b]);
     if (s_reach[b]) atomicAdd(&o.reached[b], s_reach[b]);
     if (s_edges[b]) atomicAdd(&o.edges[b], s_edges[b]);
   }
 }
 
-int ensure_ms(dawn_graph* g) {
+// Lane words per vertex for a traversal of S sources: the knob, but no
+// wider than the sources need.
+int words_for(const dawn_graph* g, int32_t S) {
+  const int knob = std::max(1, std::min(kMaxWords, (int)g->knobs.lane_words));
+  int w = 1;
+  while (w < knob && 64 * w < S) w <<= 1;
+  return w;
+}
+
+int ensure_ms(dawn_graph* g, int W) {
   if (!g->ms) g->ms = new MsWorkspace();
   MsWorkspace* ws = g->ms;
-  if (ws->n == g->n && ws->seen.p) return DAWN_OK;
+  if (ws->n == g->n && ws->words == W && ws->seen.p) return DAWN_OK;
+  // New layout: captured graphs hold the old buffers' addresses.
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  ws->graphs.clear();
+  ws->seen_bak.release();
+  ws->distbuf.release();
+  for (DevBuf<uint64_t>* b : {&ws->seen, &ws->next, &ws->hist}) b->release();
   ws->n = g->n;
+  ws->words = W;
   ws->npad = ((int64_t)g->n + 63) / 64 * 64;
-  DAWN_TRY(ws->seen.alloc(ws->npad));
-  DAWN_TRY(ws->next.alloc(ws->npad));
+  const size_t words = (size_t)ws->npad * W;
+  DAWN_TRY(ws->seen.alloc(words));
+  DAWN_TRY(ws->next.alloc(words));
   DAWN_TRY(ws->fbits.alloc((size_t)2 * (ws->npad / 32)));
-  DAWN_TRY(ws->hist.alloc((size_t)kHist * ws->npad));
+  DAWN_TRY(ws->hist.alloc((size_t)kHist * words));
   // Queue entries per level: one per kPushChunk out-edges of each frontier
   // vertex, at most n + m / kPushChunk.
   ws->qcap = (int64_t)g->n + g->m / kPushChunk + 64;
-  DAWN_TRY(ws->q.alloc((size_t)(2 * ws->qcap)));
-  DAWN_TRY(ws->qb.alloc((size_t)(2 * ws->qcap)));
+  DAWN_TRY(ws->q.alloc((size_t)ws->qcap));
+  DAWN_TRY(ws->qb.alloc((size_t)ws->qcap));
   DAWN_TRY(ws->ring.alloc(kRing));
   DAWN_TRY(ws->ring_bak.alloc(kRing));
   DAWN_TRY(ws->st.alloc(1));
   DAWN_TRY(ws->batch.alloc(1));
   DAWN_TRY(ws->trace_feat.alloc((size_t)kTraceMax * 8));
   DAWN_TRY(ws->trace_mode.alloc(kTraceMax));
-  DAWN_TRY(ws->lane_stats.alloc(3 * kLanes));
-  DAWN_HIP(hipMemset(ws->next.p, 0, sizeof(uint64_t) * ws->npad));
+  DAWN_TRY(ws->lane_stats.alloc(3 * kMaxLanes));
+  DAWN_HIP(hipMemset(ws->next.p, 0, sizeof(uint64_t) * words));
   if (!ws->h_state) DAWN_HIP(hipHostMalloc((void**)&ws->h_state, sizeof(SchedState)));
   if (!ws->h_batch) DAWN_HIP(hipHostMalloc((void**)&ws->h_batch, sizeof(MsBatch)));
   for (hipEvent_t* e : {&ws->ev0, &ws->ev1, &ws->e_a, &ws->e_b})
@@ -702,7 +926,7 @@ MsArgs make_args(dawn_graph* g) {
   a.oc_end = g->oc_end;
   a.num_oc = g->num_oc;
   a.heavy = g->knobs.heavy_degree;
-  a.balanced = g->knobs.pull_balanced;
+  a.variant = g->knobs.pull_variant;
   a.deg_bits = 32 - __builtin_clz((unsigned)std::max(1, g->max_out_degree));
   a.ring = ws->ring.p;
   a.st = ws->st.p;
@@ -726,13 +950,13 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
   return p;
 }
 
-// Dense step kernels: at most 8 workgroups per CU (2048), grid-stride beyond.
+// Dense step kernels: grid capped by knobs.dense_blocks, grid-stride beyond.
 unsigned ms_grid(const dawn_graph* g) {
   return grid_for(std::max<int64_t>(g->n, 1), kBlock, (unsigned)g->knobs.dense_blocks);
 }
-// Distance pass: 4 vertices per thread.
-unsigned ms_final_grid(const dawn_graph* g) {
-  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4, (unsigned)g->knobs.dense_blocks);
+// Distance pass: 1024/W vertices per workgroup.
+unsigned ms_final_grid(const dawn_graph* g, int W) {
+  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, (unsigned)g->knobs.dense_blocks);
 }
 
 // Stages one batch's sources (taken mod n) into the pinned batch block.
@@ -741,7 +965,10 @@ void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
   std::memset(&b, 0, sizeof b);
   b.nlanes = lanes;
   b.lo = 0;
-  b.lanemask = lanes >= 64 ? ~0ull : ((1ull << lanes) - 1);
+  for (int i = 0; i < kMaxWords; i++) {
+    const int l = lanes - 64 * i;
+    b.lanemask[i] = l >= 64 ? ~0ull : l <= 0 ? 0ull : ((1ull << l) - 1);
+  }
   for (int i = 0; i < lanes; i++) {
     int64_t s = sources[i] % (int64_t)ws->n;
     if (s < 0) s += ws->n;
@@ -749,54 +976,54 @@ void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
   }
 }
 
-}  // namespace
+template <int W>
+void launch_step(const MsArgs& a, unsigned grid, hipStream_t s) {
+  ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
+  ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
+  ms_finalize_kernel<W><<<grid, kBlock, 0, s>>>(a);
+}
 
-// Runs the batches of an unweighted multi-source traversal.
+// Runs the batches of an unweighted multi-source traversal, 64·W sources
+// per batch.
 //   dist: caller buffer (S*n int32, host/device) or null
 //   sums/reached/edges: per source (host) or null
-// mode_trace (optional): receives per-step modes of every batch.
-int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
-              int64_t* sums, int64_t* reached, int64_t* edges, int32_t policy,
-              dawn_stats* stats) {
-  if (!g || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
-  std::lock_guard<std::mutex> guard(g->lock);
-  DAWN_HIP(hipSetDevice(g->device));
-  DAWN_TRY(ensure_ms(g));
+template <int W>
+int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+                int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+  constexpr int kB = 64 * W;
+  DAWN_TRY(ensure_ms(g, W));
   MsWorkspace* ws = g->ms;
   hipStream_t s = g->stream;
   const bool dist_dev = dist && is_device_pointer(dist);
-  if (!dist_dev) DAWN_TRY(ws->distbuf.alloc((size_t)kLanes * g->n));
+  if (!dist_dev) DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
 
   MsArgs a = make_args(g);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
-  const unsigned fgrid = ms_final_grid(g);
+  const unsigned fgrid = ms_final_grid(g, W);
   unsigned long long* ls = ws->lane_stats.p;
 
   // Graph of one batch: fill, seed, kChunk steps, final pass (no-op unless
   // the traversal finished within the chunk).  Sources and the output
   // pointer live in the device batch block, so one graph serves all batches.
   auto get_graph = [&](int want_edges, hipGraphExec_t* exec) -> int {
-    const std::pair<long long, const void*> key{((long long)policy << 8) | want_edges,
-                                                (const void*)p.mlp};
+    const auto key = std::make_tuple(W, (int)policy, want_edges, (const void*)p.mlp);
     auto it = ws->graphs.find(key);
     if (it != ws->graphs.end()) {
       *exec = it->second;
       return DAWN_OK;
     }
-    MsOut o{ls, ls + kLanes, ls + 2 * kLanes, want_edges};
+    MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, want_edges};
     hipGraph_t graph;
     DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o);
-    ms_seed_kernel<<<1, 64, 0, s>>>(a);
+    ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o);
+    ms_seed_kernel<W><<<1, 64, 0, s>>>(a);
     for (int i = 0; i < kChunk; i++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
-      ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
-      ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
-      ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
+      launch_step<W>(a, grid, s);
     }
-    ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
+    ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
     DAWN_HIP(hipStreamEndCapture(s, &graph));
     hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -806,11 +1033,11 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   };
 
   const int want_edges = (stats || edges) ? 1 : 0;
-  std::vector<unsigned long long> h_ls(3 * kLanes);
+  std::vector<unsigned long long> h_ls(3 * kMaxLanes);
   std::vector<int32_t> h_modes(kTraceMax);
   float total_ms = 0.f;
-  for (int32_t b0 = 0; b0 < S; b0 += kLanes) {
-    const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
+  for (int32_t b0 = 0; b0 < S; b0 += kB) {
+    const int32_t lanes = std::min<int32_t>(kB, S - b0);
     int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : ws->distbuf.p;
     hipGraphExec_t exec;
     DAWN_TRY(get_graph(want_edges, &exec));
@@ -827,30 +1054,28 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     int lo = 0;
     int64_t guard_steps = kChunk;
     while (!ws->h_state->done) {
-      MsOut o{ls, ls + kLanes, ls + 2 * kLanes, want_edges};
+      MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, want_edges};
       const int stp = ws->h_state->step;
       if (stp > lo) {
-        ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, stp, 0);
+        ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, stp, 0);
         lo = stp;
       }
       for (int i = 0; i < kChunk; i++) {
         ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
-        ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
-        ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
-        ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
+        launch_step<W>(a, grid, s);
       }
       guard_steps += kChunk;
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
       DAWN_HIP(hipStreamSynchronize(s));
       if (ws->h_state->done) {
-        ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels, 1);
+        ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels, 1);
         break;
       }
       if (guard_steps > (int64_t)g->n + 2 * kChunk) return DAWN_ERR_HIP;
     }
     DAWN_HIP(hipGetLastError());
     DAWN_HIP(hipEventRecord(ws->ev1, s));
-    DAWN_HIP(hipMemcpyAsync(h_ls.data(), ls, sizeof(unsigned long long) * 3 * kLanes,
+    DAWN_HIP(hipMemcpyAsync(h_ls.data(), ls, sizeof(unsigned long long) * 3 * kMaxLanes,
                             hipMemcpyDeviceToHost, s));
     const int levels_now = ws->h_state->levels;
     if (stats && levels_now > 0)
@@ -868,14 +1093,14 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     }
     for (int i = 0; i < lanes; i++) {
       if (sums) sums[b0 + i] = (int64_t)h_ls[i];
-      if (reached) reached[b0 + i] = (int64_t)h_ls[kLanes + i];
-      if (edges) edges[b0 + i] = (int64_t)h_ls[2 * kLanes + i];
+      if (reached) reached[b0 + i] = (int64_t)h_ls[kMaxLanes + i];
+      if (edges) edges[b0 + i] = (int64_t)h_ls[2 * kMaxLanes + i];
     }
     if (stats) {
       stats->batches += 1;
       for (int i = 0; i < lanes; i++) {
-        stats->vertices_reached += (int64_t)h_ls[kLanes + i];
-        stats->edges_reached += (int64_t)h_ls[2 * kLanes + i];
+        stats->vertices_reached += (int64_t)h_ls[kMaxLanes + i];
+        stats->edges_reached += (int64_t)h_ls[2 * kMaxLanes + i];
         const int32_t sv = ws->h_batch->src[i];
         if (g->h_row_ptr[sv] == g->h_row_ptr[sv + 1]) stats->skipped_sources += 1;
       }
@@ -894,29 +1119,29 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
 // Per-step mode timings from identical state (scheduler training data).
 // Each step: decide (features), snapshot, then every mode is run `repeats`
 // times from the snapshot and timed with HIP events; the minimum is kept.
-int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
-                  float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
-                  int32_t* rows_out, int32_t policy) {
-  if (!g || S <= 0 || !sources || repeats < 1) return DAWN_ERR_INVALID_ARG;
-  std::lock_guard<std::mutex> guard(g->lock);
-  DAWN_HIP(hipSetDevice(g->device));
-  DAWN_TRY(ensure_ms(g));
+template <int W>
+int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
+                    float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
+                    int32_t* rows_out, int32_t policy) {
+  constexpr int kB = 64 * W;
+  DAWN_TRY(ensure_ms(g, W));
   MsWorkspace* ws = g->ms;
-  DAWN_TRY(ws->seen_bak.alloc(ws->npad));
+  const size_t words = (size_t)ws->npad * W;
+  DAWN_TRY(ws->seen_bak.alloc(words));
   hipStream_t s = g->stream;
   MsArgs a = make_args(g);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
   unsigned long long* ls = ws->lane_stats.p;
-  MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 0};
+  MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, 0};
   int32_t rows = 0;
-  for (int32_t b0 = 0; b0 < S && rows < max_rows; b0 += kLanes) {
-    const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
+  for (int32_t b0 = 0; b0 < S && rows < max_rows; b0 += kB) {
+    const int32_t lanes = std::min<int32_t>(kB, S - b0);
     stage_batch(ws, sources + b0, lanes);
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
-    ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o);
-    ms_seed_kernel<<<1, 64, 0, s>>>(a);
+    ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o);
+    ms_seed_kernel<W><<<1, 64, 0, s>>>(a);
     for (int step = 1; rows < max_rows && step < kHist; step++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
@@ -926,7 +1151,7 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
       if (chosen) chosen[rows] = snap.mode;
       DAWN_HIP(hipMemcpyAsync(features + (size_t)rows * 8, ws->trace_feat.p + (size_t)snap.step * 8,
                               sizeof(float) * 8, hipMemcpyDeviceToHost, s));
-      DAWN_HIP(hipMemcpyAsync(ws->seen_bak.p, ws->seen.p, sizeof(uint64_t) * ws->npad,
+      DAWN_HIP(hipMemcpyAsync(ws->seen_bak.p, ws->seen.p, sizeof(uint64_t) * words,
                               hipMemcpyDeviceToDevice, s));
       DAWN_HIP(hipMemcpyAsync(ws->ring_bak.p, ws->ring.p, sizeof(LevelCounters) * kRing,
                               hipMemcpyDeviceToDevice, s));
@@ -937,14 +1162,12 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
           st2.mode = mode;
           *ws->h_state = st2;
           DAWN_HIP(hipMemcpyAsync(ws->st.p, ws->h_state, sizeof(SchedState), hipMemcpyHostToDevice, s));
-          DAWN_HIP(hipMemcpyAsync(ws->seen.p, ws->seen_bak.p, sizeof(uint64_t) * ws->npad,
+          DAWN_HIP(hipMemcpyAsync(ws->seen.p, ws->seen_bak.p, sizeof(uint64_t) * words,
                                   hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipMemcpyAsync(ws->ring.p, ws->ring_bak.p, sizeof(LevelCounters) * kRing,
                                   hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipEventRecord(ws->e_a, s));
-          ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
-          ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
-          ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a);
+          launch_step<W>(a, grid, s);
           DAWN_HIP(hipEventRecord(ws->e_b, s));
           DAWN_HIP(hipEventSynchronize(ws->e_b));
           float ms = 0.f;
@@ -961,35 +1184,33 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
   return DAWN_OK;
 }
 
-}  // namespace dawn
-
-namespace dawn {
-
 // Same kernels as run_msbfs, launched one by one with HIP events around each
-// launch on the graph's stream.  kernel_ms / launches are indexed by
-// DAWN_PROF_* (fill, seed, decide, expand push, expand pull, expand bitmap,
-// finalize, final).  levels_out: per batch number of levels (may be null).
-int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
-                   double* kernel_ms, int64_t* launches, int32_t* levels_out) {
-  if (!g || S <= 0 || !sources || !kernel_ms || !launches) return DAWN_ERR_INVALID_ARG;
-  std::lock_guard<std::mutex> guard(g->lock);
-  DAWN_HIP(hipSetDevice(g->device));
-  DAWN_TRY(ensure_ms(g));
+// launch on the graph's stream, writing distance rows and statistics like a
+// timed traversal.  kernel_ms / launches are indexed by DAWN_PROF_* (fill,
+// seed, decide, expand push, expand pull, expand bitmap, finalize, final).
+// levels_out: number of levels of each batch of 64·W sources, -1 past the
+// last batch (one entry per 64 sources).
+template <int W>
+int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
+                     double* kernel_ms, int64_t* launches, int32_t* levels_out) {
+  constexpr int kB = 64 * W;
+  DAWN_TRY(ensure_ms(g, W));
   MsWorkspace* ws = g->ms;
-  DAWN_TRY(ws->distbuf.alloc((size_t)kLanes * g->n));
+  DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
   hipStream_t s = g->stream;
   MsArgs a = make_args(g);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
-  const unsigned fgrid = ms_final_grid(g);
+  const unsigned fgrid = ms_final_grid(g, W);
   unsigned long long* ls = ws->lane_stats.p;
-  // Same work as a traversal that writes distance rows and statistics.
-  MsOut o{ls, ls + kLanes, ls + 2 * kLanes, 1};
+  MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, 1};
   for (int i = 0; i < 8; i++) {
     kernel_ms[i] = 0;
     launches[i] = 0;
   }
+  if (levels_out)
+    for (int32_t i = 0; i < (S + 63) / 64; i++) levels_out[i] = -1;
   auto timed = [&](int slot, auto&& launch) -> int {
     DAWN_HIP(hipEventRecord(ws->e_a, s));
     launch();
@@ -1003,14 +1224,14 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
     return DAWN_OK;
   };
   int batch = 0;
-  for (int32_t b0 = 0; b0 < S; b0 += kLanes, batch++) {
-    const int32_t lanes = std::min<int32_t>(kLanes, S - b0);
+  for (int32_t b0 = 0; b0 < S; b0 += kB, batch++) {
+    const int32_t lanes = std::min<int32_t>(kB, S - b0);
     stage_batch(ws, sources + b0, lanes);
     ws->h_batch->dist = ws->distbuf.p;
     ws->h_batch->stride = g->n;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
-    DAWN_TRY(timed(0, [&] { ms_fill_kernel<<<grid_for(ws->npad), kBlock, 0, s>>>(a, o); }));
-    DAWN_TRY(timed(1, [&] { ms_seed_kernel<<<1, 64, 0, s>>>(a); }));
+    DAWN_TRY(timed(0, [&] { ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o); }));
+    DAWN_TRY(timed(1, [&] { ms_seed_kernel<W><<<1, 64, 0, s>>>(a); }));
     int lo = 0;
     for (int64_t guard_steps = 0;; guard_steps++) {
       if (guard_steps > (int64_t)g->n + 2) return DAWN_ERR_HIP;
@@ -1018,24 +1239,69 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
       DAWN_HIP(hipMemcpy(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost));
       const SchedState st = *ws->h_state;
       if (st.done) {
-        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.levels, 1); }));
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.levels, 1); }));
         if (levels_out) levels_out[batch] = st.levels;
         break;
       }
       const int slot = st.mode == DAWN_MODE_PUSH ? 3 : st.mode == DAWN_MODE_PULL ? 4 : 5;
       DAWN_TRY(timed(slot, [&] {
         ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
-        ms_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+        ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
       }));
-      DAWN_TRY(timed(6, [&] { ms_finalize_kernel<<<grid, kBlock, 0, s>>>(a); }));
+      DAWN_TRY(timed(6, [&] { ms_finalize_kernel<W><<<grid, kBlock, 0, s>>>(a); }));
       if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
-        DAWN_TRY(timed(7, [&] { ms_final_kernel<<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
         lo = st.step;
       }
     }
   }
   DAWN_HIP(hipStreamSynchronize(s));
   return DAWN_OK;
+}
+
+}  // namespace
+
+int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+  if (!g || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  switch (words_for(g, S)) {
+    case 1: return run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats);
+    case 2: return run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats);
+    default: return run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats);
+  }
+}
+
+int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
+                  float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
+                  int32_t* rows_out, int32_t policy) {
+  if (!g || S <= 0 || !sources || repeats < 1) return DAWN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  switch (words_for(g, S)) {
+    case 1:
+      return run_ms_record_w<1>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
+                                rows_out, policy);
+    case 2:
+      return run_ms_record_w<2>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
+                                rows_out, policy);
+    default:
+      return run_ms_record_w<4>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
+                                rows_out, policy);
+  }
+}
+
+int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
+                   double* kernel_ms, int64_t* launches, int32_t* levels_out) {
+  if (!g || S <= 0 || !sources || !kernel_ms || !launches) return DAWN_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  switch (words_for(g, S)) {
+    case 1: return run_ms_profile_w<1>(g, sources, S, policy, kernel_ms, launches, levels_out);
+    case 2: return run_ms_profile_w<2>(g, sources, S, policy, kernel_ms, launches, levels_out);
+    default: return run_ms_profile_w<4>(g, sources, S, policy, kernel_ms, launches, levels_out);
+  }
 }
 
 }  // namespace dawn

@@ -10,12 +10,13 @@
 // graph's class (log2 n, log2 average degree, directed, weighted); graphs
 // created later in that class start from the cached knobs.  The cache
 // persists as text: one line "class dense_blocks heavy_degree pull_chunk
-// ss_chunk pull_balanced".
+// ss_chunk pull_variant lane_words".
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -64,6 +65,12 @@ int sample_ms(dawn_graph* g, const std::vector<int32_t>& src, double* ms) {
   return DAWN_OK;
 }
 
+bool knobs_valid(const TuneKnobs& k) {
+  return k.dense_blocks >= 1 && k.dense_blocks <= 65536 && k.heavy_degree >= 1 &&
+         k.pull_chunk >= 1 && k.ss_chunk >= 0 && k.ss_chunk <= 1024 && k.pull_variant >= 0 &&
+         k.pull_variant <= 3 && (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4);
+}
+
 int apply(dawn_graph* g, const TuneKnobs& k) {
   std::lock_guard<std::mutex> guard(g->lock);  // traversals hold it while they run
   if (g->stream) DAWN_HIP(hipStreamSynchronize(g->stream));
@@ -97,7 +104,7 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
   if (!g || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   // Sample sources: the highest-degree vertex and spread vertices with edges.
   std::vector<int32_t> src;
-  const int want = g->weighted ? 2 : 128;
+  const int want = g->weighted ? 2 : 256;
   for (int32_t v = 0; v < g->n && (int)src.size() < want; v += std::max(1, g->n / (4 * want)))
     if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
   if (src.empty()) return DAWN_ERR_INVALID_ARG;
@@ -110,9 +117,10 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     double ms = 0;
     DAWN_TRY(sample_ms(g, src, &ms));
     if (verbose)
-      std::fprintf(stderr, "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d balanced=%d: %.3f ms\n",
+      std::fprintf(stderr,
+                   "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d variant=%d words=%d: %.3f ms\n",
                    class_key(g).c_str(), k.dense_blocks, k.heavy_degree, k.pull_chunk, k.ss_chunk,
-                   k.pull_balanced, ms);
+                   k.pull_variant, k.lane_words, ms);
     if (ms < best_ms) {
       best_ms = ms;
       best = k;
@@ -126,9 +134,15 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       DAWN_TRY(consider(k));
     }
   } else {
-    for (int pb : {0, 1}) {
+    // The same 256 sample sources at every width (4, 2 or 1 batches).
+    for (int lw : {1, 2, 4}) {
       TuneKnobs k = best;
-      k.pull_balanced = pb;
+      k.lane_words = lw;
+      DAWN_TRY(consider(k));
+    }
+    for (int pv : {0, 1, 2, 3}) {
+      TuneKnobs k = best;
+      k.pull_variant = pv;
       DAWN_TRY(consider(k));
     }
     for (int b : {512, 1024, 2048, 4096}) {
@@ -151,9 +165,10 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     g_tune_cache[class_key(g)] = best;
   }
   if (verbose)
-    std::fprintf(stderr, "tune %s -> blocks=%d heavy=%d chunk=%d ss_chunk=%d balanced=%d (%.3f ms)\n",
+    std::fprintf(stderr,
+                 "tune %s -> blocks=%d heavy=%d chunk=%d ss_chunk=%d variant=%d words=%d (%.3f ms)\n",
                  class_key(g).c_str(), best.dense_blocks, best.heavy_degree, best.pull_chunk,
-                 best.ss_chunk, best.pull_balanced, best_ms);
+                 best.ss_chunk, best.pull_variant, best.lane_words, best_ms);
   return DAWN_OK;
 }
 
@@ -166,11 +181,10 @@ int dawn_tune_load(const char* path) {
   std::map<std::string, TuneKnobs> loaded;
   int rc = DAWN_OK;
   for (;;) {
-    const int got = std::fscanf(f, "%63s %d %d %d %d %d", key, &k.dense_blocks, &k.heavy_degree,
-                                &k.pull_chunk, &k.ss_chunk, &k.pull_balanced);
+    const int got = std::fscanf(f, "%63s %d %d %d %d %d %d", key, &k.dense_blocks, &k.heavy_degree,
+                                &k.pull_chunk, &k.ss_chunk, &k.pull_variant, &k.lane_words);
     if (got == EOF) break;
-    if (got != 6 || k.dense_blocks < 1 || k.heavy_degree < 1 || k.pull_chunk < 1 || k.ss_chunk < 0 ||
-        (k.pull_balanced != 0 && k.pull_balanced != 1)) {
+    if (got != 7 || !knobs_valid(k)) {
       rc = DAWN_ERR_FORMAT;
       break;
     }
@@ -189,11 +203,33 @@ int dawn_tune_save(const char* path) {
   if (!f) return DAWN_ERR_IO;
   std::lock_guard<std::mutex> l(g_tune_lock);
   for (auto& kv : g_tune_cache)
-    std::fprintf(f, "%s %d %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
+    std::fprintf(f, "%s %d %d %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
                  kv.second.heavy_degree, kv.second.pull_chunk, kv.second.ss_chunk,
-                 kv.second.pull_balanced);
+                 kv.second.pull_variant, kv.second.lane_words);
   std::fclose(f);
   return DAWN_OK;
+}
+
+int dawn_graph_get_knobs(const dawn_graph* g, int32_t* knobs) {
+  if (!g || !knobs) return DAWN_ERR_INVALID_ARG;
+  const TuneKnobs& k = g->knobs;
+  const int32_t v[DAWN_NUM_KNOBS] = {k.dense_blocks, k.heavy_degree, k.pull_chunk,
+                                     k.ss_chunk,     k.pull_variant, k.lane_words};
+  std::memcpy(knobs, v, sizeof v);
+  return DAWN_OK;
+}
+
+int dawn_graph_set_knobs(dawn_graph* g, const int32_t* knobs) {
+  if (!g || !knobs) return DAWN_ERR_INVALID_ARG;
+  TuneKnobs k;
+  k.dense_blocks = knobs[DAWN_KNOB_DENSE_BLOCKS];
+  k.heavy_degree = knobs[DAWN_KNOB_HEAVY_DEGREE];
+  k.pull_chunk = knobs[DAWN_KNOB_PULL_CHUNK];
+  k.ss_chunk = knobs[DAWN_KNOB_SS_CHUNK];
+  k.pull_variant = knobs[DAWN_KNOB_PULL_VARIANT];
+  k.lane_words = knobs[DAWN_KNOB_LANE_WORDS];
+  if (!knobs_valid(k)) return DAWN_ERR_INVALID_ARG;
+  return apply(g, k);
 }
 
 }  // extern "C"

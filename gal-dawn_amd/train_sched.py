@@ -1,6 +1,7 @@
 """Train the step scheduler's MLP on per-step mode timings.
 
-1. Record: on each training graph the engine runs 64-source batches; at
+1. Record: on each training graph the engine runs batches of 64*W sources
+   (W = the graph's lane_words knob, 4 by default: 256 sources); at
    every step it times PUSH, PULL and BITMAP from the same state (HIP
    events, minimum of `repeats`) and stores the step's 8 features
    (dawn_sched_record).
@@ -60,7 +61,7 @@ def record(quick: bool, repeats: int, batches: int):
     for kind, size, seed in training_graphs(quick):
         h = make(kind, size, seed)
         g = dawn.Graph(h)
-        src = dawn.gen_sources(h.n, 64 * batches, seed)
+        src = dawn.gen_sources(h.n, 64 * g.knobs()["lane_words"] * batches, seed)
         t0 = time.time()
         feat, times, fixed = g.sched_record(src, repeats=repeats)
         g.close()

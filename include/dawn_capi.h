@@ -253,12 +253,33 @@ int dawn_mlp_train(float* params, const float* x, const float* y, int32_t N,
 
 /* ------------------------------------------------------------ autotune --- */
 
-/* Autotuned launch knobs per (kernel, graph class).  dawn_tune runs a sweep
- * on g and stores the winners in the in-process cache; load/save persist
- * the cache as text. */
+/* Autotuned launch knobs per graph class (no counterpart in the reference,
+ * whose launch shape is fixed).  dawn_tune runs a sweep on g, applies the
+ * winners to g and stores them in the in-process cache, from which graphs
+ * created later in the same class start; load/save persist the cache as
+ * text, one line per class: "class dense_blocks heavy_degree pull_chunk
+ * ss_chunk pull_variant lane_words". */
 int dawn_tune(const dawn_graph* g, int32_t verbose);
 int dawn_tune_load(const char* path);
 int dawn_tune_save(const char* path);
+
+/* The knobs of one graph, in the order of DAWN_KNOB_*: grid cap of the
+ * dense batched kernels, degree from which edge lists are cut into wave
+ * chunks, chunk length, single-source steps per captured graph (0 = auto),
+ * light-pull variant (0..3), 64-lane words per vertex of the batched engine
+ * (1, 2 or 4: 64, 128 or 256 sources per batch).  Knobs change time only,
+ * never results.  dawn_graph_set_knobs rejects values out of range. */
+enum {
+  DAWN_KNOB_DENSE_BLOCKS = 0,
+  DAWN_KNOB_HEAVY_DEGREE = 1,
+  DAWN_KNOB_PULL_CHUNK = 2,
+  DAWN_KNOB_SS_CHUNK = 3,
+  DAWN_KNOB_PULL_VARIANT = 4,
+  DAWN_KNOB_LANE_WORDS = 5,
+  DAWN_NUM_KNOBS = 6
+};
+int dawn_graph_get_knobs(const dawn_graph* g, int32_t* knobs);
+int dawn_graph_set_knobs(dawn_graph* g, const int32_t* knobs);
 
 #ifdef __cplusplus
 }
