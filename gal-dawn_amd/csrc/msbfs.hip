@@ -16,9 +16,13 @@
 //   next   u64[npad][W]        scatter target of the step (kept all-zero
 //                              between steps by the finalize pass)
 //   hist   u64[H][npad][W]     level history: slot l%H holds the lanes that
-//                              reached v exactly at level l (= frontier of l)
-//   fbits  u32[2][npad/32]     1 bit per vertex: in the frontier of a level
-//                              (any lane), slot = level parity
+//                              reached v exactly at level l (= frontier of l),
+//                              valid only where fbits of that level is set
+//                              (entries of other vertices are never read)
+//   fbits  u32[H][npad/32]     1 bit per vertex: in the frontier of level l
+//                              (any lane), slot l%H
+//   done   u32[npad/32]        1 bit per vertex: no live lane is missing (stays
+//                              set: seen only grows, live lanes only shrink)
 //   q, qb  int32[qcap]         push queue: one entry per 256 out-edges of a
 //                              frontier vertex (vertex, first edge)
 //   dist   int32[64W][n]       output rows of the batch (caller or workspace)
@@ -33,7 +37,8 @@
 //           atomicOr of (frontier & ~seen[v]) into next[v], word by word.
 //   BITMAP  dense scan of the frontier bitmap, same scatter as PUSH.
 //   Every step ends with a finalize pass that folds next into the level
-//   slot, seen, the frontier bitmap and the level counters.
+//   slot, seen, the frontier bitmap and the level counters; vertices marked
+//   done are skipped, and level words are written only for new arrivals.
 // After the last level one pass turns the history into int32 distances
 // (16-byte row stores) and per-lane sums/counts (64x64 bit transposes +
 // popcount), so no distance is written twice.
@@ -77,7 +82,8 @@ struct MsArgs {
   uint64_t* seen;
   uint64_t* next;
   uint64_t* hist;
-  uint32_t* fbits;     // [2][fwords]
+  uint32_t* fbits;     // [H][fwords]
+  uint32_t* done;      // [fwords]
   int64_t fwords;
   int32_t* q;          // [qcap] frontier entries: vertex
   int32_t* qb;         // [qcap] first out-edge of the entry's chunk
@@ -110,7 +116,7 @@ struct MsWorkspace {
   int32_t words = 0;   // W the buffers are laid out for
   int64_t npad = 0;
   DevBuf<uint64_t> seen, next, hist, seen_bak;
-  DevBuf<uint32_t> fbits;
+  DevBuf<uint32_t> fbits, done, done_bak;
   DevBuf<int32_t> q, qb, distbuf;
   int64_t qcap = 0;
   DevBuf<LevelCounters> ring, ring_bak;
@@ -237,11 +243,12 @@ __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < words; x += gs) {
     const int64_t v = x / W;
     a.seen[x] = v < a.n ? ~b.lanemask[x % W] : ~0ull;
-    a.hist[x] = 0ull;  // level-0 slot
     a.next[x] = 0ull;
   }
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.fwords; w += gs)
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.fwords; w += gs) {
     a.fbits[w] = 0u;  // level-0 frontier bitmap
+    a.done[w] = 0u;
+  }
   if (blockIdx.x == 0 && threadIdx.x < kRing) a.ring[threadIdx.x] = LevelCounters{};
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 64 * W; t += gs) {
     o.sum[t] = 0;
@@ -260,9 +267,8 @@ __global__ void ms_seed_kernel(MsArgs a) {
     const int32_t v = b.src[l];
     const int i = l >> 6;
     const uint64_t bit = 1ull << (l & 63);
-    uint64_t any = 0;
-    for (int k = 0; k < W; k++) any |= a.hist[(int64_t)v * W + k];
-    if (!any) {
+    if (!((a.fbits[v >> 5] >> (v & 31)) & 1u)) {  // first lane at v
+      for (int k = 0; k < W; k++) a.hist[(int64_t)v * W + k] = 0ull;
       const int d = out_deg(a, v);
       a.fbits[v >> 5] |= 1u << (v & 31);
       c.fv += 1;
@@ -417,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   } else if (st.mode == DAWN_MODE_BITMAP) {
     // Dense top-down: light frontier vertices one lane each, heavy ones in
     // out-edge chunks of knobs.pull_chunk edges, one wave per chunk.
-    const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
+    const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
@@ -441,6 +447,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t c = wave; c < a.num_oc; c += nwaves) {
       const int32_t u = a.oc_v[c];
+      if (!((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
       uint64_t f[W];
       load_words<W>(prev + (int64_t)u * W, f);
       uint64_t any = 0;
@@ -465,7 +472,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     for (int k = 0; k < W; k++) alive[k] = a.ring[(lv - 1) % kRing].alive[k];
     // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
     // frontier words are gathered only for in-neighbours that are in it.
-    const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
+    const uint32_t* fb = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     if (a.variant == 1) {
       // Light vertices, edge-balanced: the workgroup's 256 vertices' in-edge
@@ -573,19 +580,21 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
   }
 }
 
-// Folds next[] into the level slot, seen, the frontier bitmap and the
-// counters.  Each thread takes four consecutive words (4/W vertices) with
-// two 16-byte loads per array; npad is a multiple of 64, so a thread's
-// vertices never straddle it.
+// Folds next[] into the level slot, seen, the frontier bitmap, the done
+// bitmap and the counters.  Each thread takes four consecutive words (4/W
+// vertices) with two 16-byte loads per array; npad is a multiple of 64, so
+// a thread's vertices never straddle it.  Threads whose vertices are all
+// done read one bitmap word and nothing else.
 template <int W>
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   constexpr int V = 4 / W;     // vertices per thread
   constexpr int LPW = 32 / V;  // lanes per 32-bit bitmap word
+  constexpr unsigned kAll = (1u << V) - 1;
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
   uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
-  uint32_t* fb = a.fbits + (int64_t)(lv & 1) * a.fwords;
+  uint32_t* fb = a.fbits + (int64_t)(lv % a.H) * a.fwords;
   LevelCounters* c_out = a.ring + (lv % kRing);
   uint64_t alive_prev[W];
 #pragma unroll
@@ -595,8 +604,11 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   constexpr int64_t kTile = (int64_t)kBlock * V;
   for (int64_t b0 = (int64_t)blockIdx.x * kTile; b0 < a.npad; b0 += (int64_t)gridDim.x * kTile) {
     const int64_t v0 = b0 + (int64_t)threadIdx.x * V;
-    unsigned nib = 0;
-    if (v0 < a.npad) {
+    unsigned nib = 0, dnib = kAll;
+    if (v0 < a.npad) dnib = (a.done[v0 >> 5] >> (v0 & 31)) & kAll;
+    if (v0 < a.npad && dnib != kAll) {
+      // A done vertex has next == 0 (no step writes lanes it already has)
+      // and is complete, so the code below leaves it unchanged.
       const int64_t w0 = v0 * W;
       ulonglong2* np2 = reinterpret_cast<ulonglong2*>(a.next + w0);
       ulonglong2* sp2 = reinterpret_cast<ulonglong2*>(a.seen + w0);
@@ -610,10 +622,16 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       for (int k = 0; k < 4; k++) nw[k] = nx[k] & ~sv[k];
       if (nx[0] | nx[1]) np2[0] = make_ulonglong2(0ull, 0ull);
       if (nx[2] | nx[3]) np2[1] = make_ulonglong2(0ull, 0ull);
-      cp2[0] = make_ulonglong2(nw[0], nw[1]);
-      cp2[1] = make_ulonglong2(nw[2], nw[3]);
-      if (nw[0] | nw[1]) sp2[0] = make_ulonglong2(sv[0] | nw[0], sv[1] | nw[1]);
-      if (nw[2] | nw[3]) sp2[1] = make_ulonglong2(sv[2] | nw[2], sv[3] | nw[3]);
+      // Level words only where something arrived (the frontier bitmap
+      // marks which entries of the slot are valid).
+      if (nw[0] | nw[1]) {
+        cp2[0] = make_ulonglong2(nw[0], nw[1]);
+        sp2[0] = make_ulonglong2(sv[0] | nw[0], sv[1] | nw[1]);
+      }
+      if (nw[2] | nw[3]) {
+        cp2[1] = make_ulonglong2(nw[2], nw[3]);
+        sp2[1] = make_ulonglong2(sv[2] | nw[2], sv[3] | nw[3]);
+      }
       // Padding vertices (v >= n) have seen = ~0 and next = 0: never new,
       // never unvisited.
 #pragma unroll
@@ -640,14 +658,23 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
         if (unv) {  // still misses a live lane
           acc.uv += 1;
           acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+        } else {
+          dnib |= 1u << j;
         }
       }
     }
-    // Frontier bitmap of the level: LPW lanes (32 vertices) per word.
+    // Frontier and done bitmaps: LPW lanes (32 vertices) per word.
     unsigned x = nib << (V * (lane % LPW));
+    unsigned y = dnib << (V * (lane % LPW));
 #pragma unroll
-    for (int o = 1; o < LPW; o <<= 1) x |= __shfl_xor(x, o);
-    if ((lane % LPW) == 0 && v0 < a.npad) fb[v0 >> 5] = x;
+    for (int o = 1; o < LPW; o <<= 1) {
+      x |= __shfl_xor(x, o);
+      y |= __shfl_xor(y, o);
+    }
+    if ((lane % LPW) == 0 && v0 < a.npad) {
+      fb[v0 >> 5] = x;
+      a.done[v0 >> 5] = y;
+    }
   }
   acc.flush(c_out);
 }
@@ -661,7 +688,7 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
   const SchedState st = *a.st;
   if (st.done || st.mode != DAWN_MODE_PUSH) return;
   const int lv = st.step;
-  const uint32_t* fb = a.fbits + (int64_t)((lv - 1) & 1) * a.fwords;
+  const uint32_t* fb = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
   int32_t* q = a.q;
   int32_t* qb = a.qb;
   unsigned* qlen = &a.ring[(lv - 1) % kRing].qlen;
@@ -743,15 +770,21 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         for (int j = 0; j < 4; j++) sv[j] = a.seen[(v0 + j) * W + wi];
       }
       for (int l = lo; l < hi; l++) {
+        // Level words are valid only for the level's frontier vertices.
+        const unsigned fm = (a.fbits[(int64_t)(l % a.H) * a.fwords + (v0 >> 5)] >> (v0 & 31)) & 15u;
+        if (!fm) continue;
         const uint64_t* hb = a.hist + (int64_t)(l % a.H) * a.npad * W;
         uint64_t w[4];
         if constexpr (W == 1) {
           const ulonglong2* hp = reinterpret_cast<const ulonglong2*>(hb + v0);
           const ulonglong2 h01 = hp[0], h23 = hp[1];
           w[0] = h01.x; w[1] = h01.y; w[2] = h23.x; w[3] = h23.y;
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (!((fm >> j) & 1u)) w[j] = 0;
         } else {
 #pragma unroll
-          for (int j = 0; j < 4; j++) w[j] = hb[(v0 + j) * W + wi];
+          for (int j = 0; j < 4; j++) w[j] = ((fm >> j) & 1u) ? hb[(v0 + j) * W + wi] : 0ull;
         }
         const int r = l - lo;
 #pragma unroll
@@ -768,6 +801,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         if (v >= a.n) continue;
         sv[j] = a.seen[v * W + wi];
         for (int l = lo; l < hi; l++) {
+          if (!((a.fbits[(int64_t)(l % a.H) * a.fwords + (v >> 5)] >> (v & 31)) & 1u)) continue;
           const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad * W + v * W + wi];
           const int r = l - lo;
           F[j] |= w;
@@ -876,7 +910,8 @@ int ensure_ms(dawn_graph* g, int W) {
   const size_t words = (size_t)ws->npad * W;
   DAWN_TRY(ws->seen.alloc(words));
   DAWN_TRY(ws->next.alloc(words));
-  DAWN_TRY(ws->fbits.alloc((size_t)2 * (ws->npad / 32)));
+  DAWN_TRY(ws->fbits.alloc((size_t)kHist * (ws->npad / 32)));
+  DAWN_TRY(ws->done.alloc((size_t)(ws->npad / 32)));
   DAWN_TRY(ws->hist.alloc((size_t)kHist * words));
   // Queue entries per level: one per kPushChunk out-edges of each frontier
   // vertex, at most n + m / kPushChunk.
@@ -912,6 +947,7 @@ MsArgs make_args(dawn_graph* g) {
   a.seen = ws->seen.p;
   a.next = ws->next.p;
   a.fbits = ws->fbits.p;
+  a.done = ws->done.p;
   a.fwords = ws->npad / 32;
   a.hist = ws->hist.p;
   a.q = ws->q.p;
@@ -954,6 +990,11 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
 unsigned ms_grid(const dawn_graph* g) {
   return grid_for(std::max<int64_t>(g->n, 1), kBlock, (unsigned)g->knobs.dense_blocks);
 }
+// Finalize pass: 1024/W vertices per workgroup and iteration; at most 2048
+// workgroups (8 per CU), since each flushes its counters with atomics.
+unsigned ms_fin_grid(const dawn_graph* g, int W) {
+  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, 2048u);
+}
 // Distance pass: 1024/W vertices per workgroup.
 unsigned ms_final_grid(const dawn_graph* g, int W) {
   return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, (unsigned)g->knobs.dense_blocks);
@@ -977,10 +1018,10 @@ void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
 }
 
 template <int W>
-void launch_step(const MsArgs& a, unsigned grid, hipStream_t s) {
+void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t s) {
   ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
   ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
-  ms_finalize_kernel<W><<<grid, kBlock, 0, s>>>(a);
+  ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
 }
 
 // Runs the batches of an unweighted multi-source traversal, 64·W sources
@@ -1001,6 +1042,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
+  const unsigned fin_grid = ms_fin_grid(g, W);
   const unsigned fgrid = ms_final_grid(g, W);
   unsigned long long* ls = ws->lane_stats.p;
 
@@ -1021,7 +1063,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     ms_seed_kernel<W><<<1, 64, 0, s>>>(a);
     for (int i = 0; i < kChunk; i++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
-      launch_step<W>(a, grid, s);
+      launch_step<W>(a, grid, fin_grid, s);
     }
     ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
     DAWN_HIP(hipStreamEndCapture(s, &graph));
@@ -1062,7 +1104,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       }
       for (int i = 0; i < kChunk; i++) {
         ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
-        launch_step<W>(a, grid, s);
+        launch_step<W>(a, grid, fin_grid, s);
       }
       guard_steps += kChunk;
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
@@ -1128,11 +1170,14 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
   MsWorkspace* ws = g->ms;
   const size_t words = (size_t)ws->npad * W;
   DAWN_TRY(ws->seen_bak.alloc(words));
+  DAWN_TRY(ws->done_bak.alloc((size_t)(ws->npad / 32)));
+  const size_t done_bytes = sizeof(uint32_t) * (size_t)(ws->npad / 32);
   hipStream_t s = g->stream;
   MsArgs a = make_args(g);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
+  const unsigned fin_grid = ms_fin_grid(g, W);
   unsigned long long* ls = ws->lane_stats.p;
   MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, 0};
   int32_t rows = 0;
@@ -1155,6 +1200,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
                               hipMemcpyDeviceToDevice, s));
       DAWN_HIP(hipMemcpyAsync(ws->ring_bak.p, ws->ring.p, sizeof(LevelCounters) * kRing,
                               hipMemcpyDeviceToDevice, s));
+      DAWN_HIP(hipMemcpyAsync(ws->done_bak.p, ws->done.p, done_bytes, hipMemcpyDeviceToDevice, s));
       for (int mode = 0; mode < DAWN_NUM_MODES; mode++) {
         float best = 1e30f;
         for (int r = 0; r < repeats; r++) {
@@ -1166,8 +1212,9 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
                                   hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipMemcpyAsync(ws->ring.p, ws->ring_bak.p, sizeof(LevelCounters) * kRing,
                                   hipMemcpyDeviceToDevice, s));
+          DAWN_HIP(hipMemcpyAsync(ws->done.p, ws->done_bak.p, done_bytes, hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipEventRecord(ws->e_a, s));
-          launch_step<W>(a, grid, s);
+          launch_step<W>(a, grid, fin_grid, s);
           DAWN_HIP(hipEventRecord(ws->e_b, s));
           DAWN_HIP(hipEventSynchronize(ws->e_b));
           float ms = 0.f;
@@ -1202,6 +1249,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
+  const unsigned fin_grid = ms_fin_grid(g, W);
   const unsigned fgrid = ms_final_grid(g, W);
   unsigned long long* ls = ws->lane_stats.p;
   MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, 1};
@@ -1248,7 +1296,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
         ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
         ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
       }));
-      DAWN_TRY(timed(6, [&] { ms_finalize_kernel<W><<<grid, kBlock, 0, s>>>(a); }));
+      DAWN_TRY(timed(6, [&] { ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a); }));
       if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
         DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
         lo = st.step;

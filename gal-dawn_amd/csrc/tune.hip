@@ -145,7 +145,7 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       k.pull_variant = pv;
       DAWN_TRY(consider(k));
     }
-    for (int b : {512, 1024, 2048, 4096}) {
+    for (int b : {1024, 2048, 4096, 8192, 16384}) {
       TuneKnobs k = best;
       k.dense_blocks = b;
       DAWN_TRY(consider(k));
