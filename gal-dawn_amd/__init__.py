@@ -60,6 +60,18 @@ class _Csr(ctypes.Structure):
     ]
 
 
+class Frontier(ctypes.Structure):
+    """dawn_frontier: device pointers of one step (include/dawn_capi.h)."""
+    _fields_ = [
+        ("alpha", ctypes.c_void_p),
+        ("beta", ctypes.c_void_p),
+        ("dist", ctypes.c_void_p),
+        ("changed", ctypes.c_void_p),
+        ("level", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("device_ms", ctypes.c_double),
@@ -121,6 +133,7 @@ def _sig(lib):
         "dawn_batch_mssp": (c.c_int, [c.POINTER(P), c.c_int32, _I32P, c.c_int32, P, _I64P, c.POINTER(Stats)]),
         "dawn_probe_copy": (c.c_int, [c.c_int64, c.c_int32, _F64P]),
         "dawn_mssp_profile": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F64P, _I64P, _I32P]),
+        "dawn_step": (c.c_int, [P, c.POINTER(Frontier), c.c_int32, c.c_void_p]),
         "dawn_sched_set_mlp": (c.c_int, [_F32P, _F32P, _F32P]),
         "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, _I32P, c.c_int32,
                                         _I32P]),
@@ -442,6 +455,17 @@ class Graph:
         # One entry per batch of 64*W sources, -1 past the last batch.
         return ({k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PROF_NAMES)},
                 levels[levels >= 0])
+
+    def step(self, alpha, beta, dist, changed, level: int = 0, mode="push", stream=None):
+        """One frontier step on device tensors (dawn_step): alpha/beta are
+        int32 bitmaps of ceil(n/32) words, dist int32[n], changed int64[1].
+        stream: a HIP stream handle (e.g. torch.cuda.current_stream().cuda_stream)
+        or None for the graph's own stream."""
+        f = Frontier(alpha.data_ptr(), beta.data_ptr(), dist.data_ptr(), changed.data_ptr(),
+                     int(level), 0)
+        md = {"push": MODE_PUSH, "pull": MODE_PULL}[mode] if isinstance(mode, str) else int(mode)
+        _check(lib().dawn_step(self._h, ctypes.byref(f), md, ctypes.c_void_p(stream or 0)),
+               "dawn_step")
 
     KNOB_NAMES = ("dense_blocks", "heavy_degree", "pull_chunk", "ss_chunk", "pull_variant",
                   "lane_words")

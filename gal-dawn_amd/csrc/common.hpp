@@ -219,6 +219,11 @@ __device__ __forceinline__ int wave_max_i32(int x) {
   return x;
 }
 
+__device__ __forceinline__ int wave_min_i32(int x) {
+  for (int o = 32; o >= 1; o >>= 1) x = min(x, __shfl_xor(x, o));
+  return x;
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int o) {
   unsigned lo = __shfl_xor((unsigned)(x & 0xffffffffu), o);
   unsigned hi = __shfl_xor((unsigned)(x >> 32), o);

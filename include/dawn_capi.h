@@ -190,9 +190,11 @@ int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
                     dawn_stats* stats);
 
 /* Profiling pass of dawn_mssp (unweighted): the same kernels launched one
- * at a time with HIP events around each launch on the graph's stream.
- * kernel_ms[8] / launches[8] are indexed by DAWN_PROF_*; levels receives
- * the level count of each 64-source batch (ceil(count/64) entries). */
+ * at a time with HIP events around each launch on the graph's stream,
+ * writing distance rows and statistics like a timed traversal.
+ * kernel_ms[8] / launches[8] are indexed by DAWN_PROF_*; levels
+ * (ceil(count/64) entries) receives the level count of each batch of 64·W
+ * sources (W = lane words in use), then -1 for the unused entries. */
 enum {
   DAWN_PROF_FILL = 0, DAWN_PROF_SEED = 1, DAWN_PROF_DECIDE = 2,
   DAWN_PROF_EXPAND_PUSH = 3, DAWN_PROF_EXPAND_PULL = 4,
@@ -250,6 +252,38 @@ int dawn_mlp_loss_grad(const float* params, const float* x, const float* y,
 int dawn_mlp_train(float* params, const float* x, const float* y, int32_t N,
                    int32_t epochs, int32_t batch, float lr, uint64_t seed,
                    float* final_loss);
+
+/* ---------------------------------------------------------- step level --- */
+
+/* One frontier step on caller-owned device state: the step-level
+ * equivalent of the reference's SOVMP (BOVM dense step,
+ * src/algorithm/cpu/bfs.cpp:159-185) and GOVMP (min-plus dense step,
+ * src/algorithm/cpu/sssp.cpp:169-199), and of the CUDA-typed GPU kernel()
+ * overloads (include/dawn/algorithm/gpu/bfs.cuh:14-19, sssp.cuh:12-18),
+ * with an opaque stream instead of CUDA types.  All pointers are device
+ * memory of g's device.
+ *   alpha    frontier bitmap, bit v of word v/32: read, then cleared (as
+ *            SOVMP clears alpha[j]);
+ *   beta     next-frontier bitmap: bits of updated vertices are OR-ed in
+ *            (the caller clears it before the step);
+ *   dist     int32[n], INT32_MAX = unreached.  Unweighted: every unreached
+ *            out-neighbour of a frontier vertex gets `level`.  Weighted:
+ *            dist[v] = min(dist[v], dist[u] + w(u,v)) over frontier u;
+ *   changed  int64 counter, set to the number of vertices whose distance
+ *            changed in this step (0 = converged).
+ * mode: DAWN_MODE_PUSH (scan the frontier's out-edges) or DAWN_MODE_PULL
+ * (unweighted only: unreached vertices scan their in-edges for a frontier
+ * vertex); both give the same result.  stream: a hipStream_t, or NULL for
+ * the graph's own stream.  The call is asynchronous on that stream. */
+typedef struct dawn_frontier {
+  uint32_t* alpha;
+  uint32_t* beta;
+  int32_t* dist;
+  int64_t* changed;
+  int32_t level;
+  int32_t reserved;
+} dawn_frontier;
+int dawn_step(const dawn_graph* g, const dawn_frontier* f, int32_t mode, void* stream);
 
 /* ------------------------------------------------------------ autotune --- */
 
