@@ -134,6 +134,11 @@ def _sig(lib):
         "dawn_probe_copy": (c.c_int, [c.c_int64, c.c_int32, _F64P]),
         "dawn_mssp_profile": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F64P, _I64P, _I32P]),
         "dawn_step": (c.c_int, [P, c.POINTER(Frontier), c.c_int32, c.c_void_p]),
+        "dawn_device_malloc": (c.c_int, [c.c_int64, c.POINTER(c.c_void_p)]),
+        "dawn_device_free": (c.c_int, [c.c_void_p]),
+        "dawn_memcpy": (c.c_int, [c.c_void_p, c.c_void_p, c.c_int64]),
+        "dawn_memset": (c.c_int, [c.c_void_p, c.c_int32, c.c_int64]),
+        "dawn_graph_synchronize": (c.c_int, [P]),
         "dawn_sched_set_mlp": (c.c_int, [_F32P, _F32P, _F32P]),
         "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, _I32P, c.c_int32,
                                         _I32P]),
@@ -366,6 +371,50 @@ def _out_arg(out):
     return ctypes.c_void_p(int(out.data_ptr()))  # torch tensor
 
 
+class DeviceArray:
+    """A device buffer from dawn_device_malloc, viewed as a numpy dtype and
+    shape; data_ptr() is what the entry points take for device outputs."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(np.asarray(shape, np.int64)).tolist())
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().dawn_device_malloc(self.nbytes, ctypes.byref(p)), "dawn_device_malloc")
+        self._p = p.value
+
+    def data_ptr(self) -> int:
+        return self._p
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().dawn_memcpy(ctypes.c_void_p(self._p), ctypes.c_void_p(a.ctypes.data), self.nbytes),
+               "dawn_memcpy")
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().dawn_memcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self._p), self.nbytes),
+               "dawn_memcpy")
+        return out
+
+    def fill_bytes(self, value: int = 0):
+        _check(lib().dawn_memset(ctypes.c_void_p(self._p), int(value), self.nbytes), "dawn_memset")
+        return self
+
+    def close(self):
+        if self._p and _lib is not None:
+            _lib.dawn_device_free(ctypes.c_void_p(self._p))
+        self._p = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Graph:
     """Device-resident graph (dawn_graph): CSR (+ CSC when directed)."""
 
@@ -456,11 +505,15 @@ class Graph:
         return ({k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PROF_NAMES)},
                 levels[levels >= 0])
 
+    def synchronize(self):
+        """Waits for the work queued on the graph's stream."""
+        _check(lib().dawn_graph_synchronize(self._h), "dawn_graph_synchronize")
+
     def step(self, alpha, beta, dist, changed, level: int = 0, mode="push", stream=None):
-        """One frontier step on device tensors (dawn_step): alpha/beta are
-        int32 bitmaps of ceil(n/32) words, dist int32[n], changed int64[1].
-        stream: a HIP stream handle (e.g. torch.cuda.current_stream().cuda_stream)
-        or None for the graph's own stream."""
+        """One frontier step (dawn_step) on device buffers (DeviceArray):
+        alpha/beta uint32 bitmaps of ceil(n/32) words, dist int32[n],
+        changed int64[1].  stream: a hipStream_t handle, or None for the
+        graph's own stream (then synchronize() before reading results)."""
         f = Frontier(alpha.data_ptr(), beta.data_ptr(), dist.data_ptr(), changed.data_ptr(),
                      int(level), 0)
         md = {"push": MODE_PUSH, "pull": MODE_PULL}[mode] if isinstance(mode, str) else int(mode)

@@ -18,6 +18,8 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
                       int32_t* trace_mode, int32_t trace_max, int32_t* trace_rows);
 int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
               int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats);
+int run_wmsbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+               dawn_stats* stats);
 int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
                   float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
                   int32_t* rows_out, int32_t policy);
@@ -67,48 +69,6 @@ const float* sched_norm_device(int device) {
   return g_nets[device].norm;
 }
 
-static int mssp_weighted(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
-                         int64_t* sums, int32_t policy, dawn_stats* stats) {
-  const bool dev = dist && is_device_pointer(dist);
-  std::vector<int32_t> row;
-  DevBuf<int32_t> drow;
-  if (sums) {
-    if (dev) DAWN_TRY(drow.alloc(g->n));
-    row.resize(g->n);
-  }
-  for (int32_t i = 0; i < S; i++) {
-    int64_t src = sources[i] % (int64_t)g->n;
-    if (src < 0) src += g->n;
-    int32_t* out = dist ? dist + (int64_t)i * g->n : nullptr;
-    if (g->h_row_ptr[src] == g->h_row_ptr[src + 1]) {
-      // Isolated source: skipped like MSSP_GPU::run (mssp.cu:28-34).
-      if (stats) stats->skipped_sources += 1;
-      if (sums) sums[i] = 0;
-      if (out) {
-        std::vector<int32_t> inf(g->n, DAWN_INF);
-        inf[src] = 0;
-        DAWN_HIP(hipMemcpy(out, inf.data(), sizeof(int32_t) * g->n,
-                           dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
-      }
-      continue;
-    }
-    int32_t* target = out ? out : (sums ? row.data() : nullptr);
-    DAWN_TRY(run_single_source(g, 1, (int32_t)src, target, policy, stats, -1, nullptr, nullptr,
-                               0, nullptr));
-    if (sums) {
-      const int32_t* h = target;
-      if (dev) {
-        DAWN_HIP(hipMemcpy(row.data(), target, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost));
-        h = row.data();
-      }
-      int64_t acc = 0;
-      for (int32_t v = 0; v < g->n; v++)
-        if (h[v] != DAWN_INF) acc += h[v];
-      sums[i] = acc;
-    }
-  }
-  return DAWN_OK;
-}
 
 }  // namespace dawn
 
@@ -135,7 +95,7 @@ int dawn_mssp(const dawn_graph* gc, const int32_t* sources, int32_t count, int32
   dawn_graph* g = const_cast<dawn_graph*>(gc);
   if (!g || count < 0 || (count > 0 && !sources)) return DAWN_ERR_INVALID_ARG;
   if (count == 0) return DAWN_OK;
-  if (g->weighted) return mssp_weighted(g, sources, count, dist, sums, policy, stats);
+  if (g->weighted) return run_wmsbfs(g, sources, count, dist, sums, stats);
   return run_msbfs(g, sources, count, dist, sums, nullptr, nullptr, policy, stats);
 }
 

@@ -84,6 +84,7 @@ struct TuneKnobs {
 
 struct MsWorkspace;   // 64-lane batched engine state (msbfs.hip)
 struct SsWorkspace;   // single-source engines state (bfs.hip / sssp.hip)
+struct WmsWorkspace;  // batched weighted engine state (wmsbfs.hip)
 
 }  // namespace dawn
 
@@ -123,6 +124,7 @@ struct dawn_graph {
   std::mutex lock;                // one traversal per graph at a time
   dawn::MsWorkspace* ms = nullptr;
   dawn::SsWorkspace* ss = nullptr;
+  dawn::WmsWorkspace* wms = nullptr;
 };
 
 namespace dawn {
@@ -300,5 +302,6 @@ bool tuned_knobs_for(const dawn_graph* g, TuneKnobs* out);
 // Workspace release hooks (defined by the engines).
 void ms_workspace_free(MsWorkspace* ws);
 void ss_workspace_free(SsWorkspace* ws);
+void wms_workspace_free(WmsWorkspace* ws);
 
 }  // namespace dawn

@@ -207,6 +207,7 @@ int dawn_graph_destroy(dawn_graph* g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   ms_workspace_free(g->ms);
   ss_workspace_free(g->ss);
+  wms_workspace_free(g->wms);
   if (g->directed) {
     if (g->in_row_ptr) (void)hipFree(g->in_row_ptr);
     if (g->in_col) (void)hipFree(g->in_col);
@@ -218,6 +219,13 @@ int dawn_graph_destroy(dawn_graph* g) {
   if (g->weight) (void)hipFree(g->weight);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
+  return DAWN_OK;
+}
+
+int dawn_graph_synchronize(const dawn_graph* g) {
+  if (!g) return DAWN_ERR_INVALID_ARG;
+  DAWN_HIP(hipSetDevice(g->device));
+  if (g->stream) DAWN_HIP(hipStreamSynchronize(g->stream));
   return DAWN_OK;
 }
 

@@ -623,12 +623,16 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       if (nx[0] | nx[1]) np2[0] = make_ulonglong2(0ull, 0ull);
       if (nx[2] | nx[3]) np2[1] = make_ulonglong2(0ull, 0ull);
       // Level words only where something arrived (the frontier bitmap
-      // marks which entries of the slot are valid).
-      if (nw[0] | nw[1]) {
+      // marks which entries of the slot are valid), but always a vertex's
+      // whole record: with W = 4 both halves belong to one vertex.
+      const bool any4 = (nw[0] | nw[1] | nw[2] | nw[3]) != 0;
+      const bool half0 = W == 4 ? any4 : (nw[0] | nw[1]) != 0;
+      const bool half1 = W == 4 ? any4 : (nw[2] | nw[3]) != 0;
+      if (half0) {
         cp2[0] = make_ulonglong2(nw[0], nw[1]);
         sp2[0] = make_ulonglong2(sv[0] | nw[0], sv[1] | nw[1]);
       }
-      if (nw[2] | nw[3]) {
+      if (half1) {
         cp2[1] = make_ulonglong2(nw[2], nw[3]);
         sp2[1] = make_ulonglong2(sv[2] | nw[2], sv[3] | nw[3]);
       }

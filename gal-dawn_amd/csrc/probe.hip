@@ -39,3 +39,36 @@ extern "C" int dawn_probe_copy(int64_t bytes, int32_t iters, double* gbps) {
   *gbps = 2.0 * (double)n * 16.0 * iters / (ms * 1e-3) / 1e9;  // read + write
   return DAWN_OK;
 }
+
+extern "C" int dawn_device_malloc(int64_t bytes, void** out) {
+  if (!out || bytes < 0) return DAWN_ERR_INVALID_ARG;
+  *out = nullptr;
+  const hipError_t e = hipMalloc(out, (size_t)(bytes ? bytes : 1));
+  if (e != hipSuccess) {
+    *out = nullptr;
+    dawn::set_hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    return e == hipErrorOutOfMemory ? DAWN_ERR_OUT_OF_MEMORY : DAWN_ERR_HIP;
+  }
+  return DAWN_OK;
+}
+
+extern "C" int dawn_device_free(void* p) {
+  if (p) DAWN_HIP(hipFree(p));
+  return DAWN_OK;
+}
+
+extern "C" int dawn_memcpy(void* dst, const void* src, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return DAWN_ERR_INVALID_ARG;
+  if (bytes) DAWN_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDefault));
+  return DAWN_OK;
+}
+
+extern "C" int dawn_memset(void* p, int32_t byte_value, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && !p)) return DAWN_ERR_INVALID_ARG;
+  if (bytes) {
+    DAWN_HIP(hipMemset(p, byte_value, (size_t)bytes));
+    DAWN_HIP(hipDeviceSynchronize());
+  }
+  return DAWN_OK;
+}
+

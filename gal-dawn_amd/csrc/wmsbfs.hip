@@ -1,0 +1,351 @@
+// Batched weighted multi-source shortest paths: int32 min-plus over up to 64
+// sources at once on one HIP device.
+//
+// Replaces the per-source weighted loops of the reference,
+// DAWN::MSSP_GPU::run_Weighted (src/algorithm/gpu/mssp.cu:53-102) and
+// DAWN::APSP_GPU::run_Weighted (src/algorithm/gpu/apsp.cu:9-60) (and the
+// CPU MSSP, whose weighted sparse step GOVM reads distance[j] for the queue
+// index, src/algorithm/cpu/sssp.cpp:157; not reproduced): the sources of a
+// batch share every frontier step.  Per source the result is the reference
+// SSSP's (SSSP_CPU::SSSP, sssp.cpp:23-80): the source's listed out-edges
+// seed their weight in row order (a later duplicate overwrites an earlier
+// one), the source itself is never relaxed, and the distances are the
+// fixed point of min-plus relaxation from that state.
+//
+// Layout in HBM (per graph, reused across batches):
+//   dist   int32[n][L]   L = 8, 16, 32 or 64 lanes (the smallest power of
+//                        two >= the batch's sources, at least 8), so one
+//                        vertex's distances of all lanes share cache lines
+//   act    u64[2][n]     lanes of v improved in the step (step parity)
+//   q      int32[2][n]   vertices with any improved lane (step parity)
+// Step (push, chaotic like GOVMP): one wave per frontier vertex u; the wave
+// is split into 64/L groups of L lanes, group g takes edges g, g+64/L, ...
+// of u, lane b of a group relaxes source lane b: dist[v][b] =
+// min(dist[v][b], dist[u][b] + w) for the lanes active at u.  The improved
+// lanes of a group are OR-ed into act_next[v] with one atomic; the first
+// one appends v to the next queue.  A K-step loop is captured once into a
+// hipGraph; every kernel reads the step and the done flag from device
+// state, so the graph is replayed until the frontier empties.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace dawn {
+
+namespace {
+
+constexpr int kWChunk = 32;  // steps per captured graph
+
+struct WState {
+  int32_t step;      // parity of the current queue = step & 1
+  int32_t done;
+  unsigned qlen[2];
+};
+
+struct WBatch {
+  int32_t nlanes;
+  int32_t pad;
+  int32_t src[64];
+};
+
+struct WArgs {
+  int32_t n;
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* w;
+  int32_t* dist;        // [n][L]
+  uint64_t* act;        // [2][n]
+  int32_t* q;           // [2][n]
+  WState* st;
+  const WBatch* batch;
+};
+
+template <int L>
+__global__ void w_fill_kernel(WArgs a) {
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  const int64_t total = (int64_t)a.n * L;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs) a.dist[i] = kInf;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * (int64_t)a.n; i += gs)
+    a.act[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.st = WState{};
+}
+
+// Reference initialisation, one thread per source lane (each owns its
+// column of dist): the listed out-edges write their weight in row order,
+// then dist[source] = 0; the out-neighbours other than the source form the
+// first frontier.  Isolated sources keep an empty frontier (row INF except
+// the source), as MSSP skips them.
+template <int L>
+__global__ void w_seed_kernel(WArgs a) {
+  const int b = threadIdx.x;
+  if (blockIdx.x != 0 || b >= a.batch->nlanes) return;
+  const int32_t s = a.batch->src[b];
+  const int32_t beg = a.row_ptr[s], end = a.row_ptr[s + 1];
+  for (int32_t k = beg; k < end; k++) a.dist[(int64_t)a.col[k] * L + b] = a.w[k];
+  a.dist[(int64_t)s * L + b] = 0;
+  for (int32_t k = beg; k < end; k++) {
+    const int32_t v = a.col[k];
+    if (v == s) continue;
+    const unsigned long long old = atomicOr((unsigned long long*)&a.act[v], 1ull << b);
+    if (old == 0) a.q[atomicAdd(&a.st->qlen[0], 1u)] = v;
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
+  constexpr int G = kWave / L;  // edge groups per wave
+  const WState st = *a.st;
+  if (st.done) return;
+  const int p = st.step & 1;
+  const unsigned qn = st.qlen[p];
+  const int32_t* q_in = a.q + (int64_t)p * a.n;
+  int32_t* q_out = a.q + (int64_t)(p ^ 1) * a.n;
+  uint64_t* act_in = a.act + (int64_t)p * a.n;
+  uint64_t* act_out = a.act + (int64_t)(p ^ 1) * a.n;
+  unsigned* qlen_out = &a.st->qlen[p ^ 1];
+  const int lane = lane_id();
+  const int b = lane % L;
+  const int grp = lane / L;
+  const uint64_t gmask = L == 64 ? ~0ull : ((1ull << L) - 1);
+  const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t i = wave; i < qn; i += nwaves) {
+    const int32_t u = q_in[i];
+    const uint64_t mask = act_in[u];
+    const bool active = (mask >> b) & 1ull;
+    const int32_t du = active ? a.dist[(int64_t)u * L + b] : kInf;
+    const int beg = a.row_ptr[u], end = a.row_ptr[u + 1];
+    for (int e0 = beg; e0 < end; e0 += G) {
+      const int e = e0 + grp;
+      bool improved = false;
+      int32_t v = -1;
+      if (e < end) {
+        v = a.col[e];
+        if (active && du != kInf) {
+          const int32_t nd = du + a.w[e];
+          int32_t* dv = a.dist + (int64_t)v * L + b;
+          if (nd < *dv) improved = nd < atomicMin(dv, nd);
+        }
+      }
+      const uint64_t bal = __ballot(improved);
+      const uint64_t mine = (bal >> (grp * L)) & gmask;
+      // One atomic per group and edge; the first improvement of v in this
+      // step appends it to the next queue.
+      bool append = false;
+      if (b == 0 && mine) append = atomicOr((unsigned long long*)&act_out[v], mine) == 0ull;
+      const int pos = wave_append(qlen_out, append);
+      if (append) q_out[pos] = v;
+    }
+    if (lane == 0) act_in[u] = 0ull;  // consumed (each u is in the queue once)
+  }
+}
+
+__global__ void w_advance_kernel(WState* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->done) return;
+  const int p = st->step & 1;
+  st->qlen[p] = 0;
+  st->step += 1;
+  if (st->qlen[p ^ 1] == 0) st->done = 1;
+}
+
+// Distance rows (lane b at out + b * stride) and per-lane sums: a workgroup
+// stages 256 vertices x L lanes in LDS (one contiguous read), then each
+// wave writes its 64 vertices of every lane as one coalesced row segment.
+template <int L>
+__global__ void __launch_bounds__(kBlock) w_final_kernel(WArgs a, int32_t* out, int64_t stride,
+                                                         unsigned long long* sums) {
+  __shared__ int32_t s_tile[kBlock * L];
+  __shared__ unsigned long long s_sum[64];
+  const int nl = a.batch->nlanes;
+  if (threadIdx.x < 64) s_sum[threadIdx.x] = 0;
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t cnt = std::min<int64_t>(kBlock, a.n - b0);  // block-uniform
+    const int32_t* src = a.dist + b0 * L;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < cnt * L; i += kBlock) s_tile[i] = src[i];
+    __syncthreads();
+    const int64_t v = b0 + wv * kWave + lane;
+    const bool ok = v < a.n;
+    for (int bl = 0; bl < nl; bl++) {
+      const int32_t d = ok ? s_tile[(wv * kWave + lane) * L + bl] : kInf;
+      if (out && ok) out[(int64_t)bl * stride + v] = d;
+      const unsigned long long sm = wave_sum_u64(d != kInf ? (unsigned long long)d : 0ull);
+      if (lane == 0 && sm) atomicAdd(&s_sum[bl], sm);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nl && s_sum[threadIdx.x]) atomicAdd(&sums[threadIdx.x], s_sum[threadIdx.x]);
+}
+
+__global__ void w_zero_kernel(unsigned long long* p, int n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+}  // namespace
+
+struct WmsWorkspace {
+  int32_t n = 0;
+  int32_t lanes = 0;
+  DevBuf<int32_t> dist, q, rows;
+  DevBuf<uint64_t> act;
+  DevBuf<WState> st;
+  DevBuf<WBatch> batch;
+  DevBuf<unsigned long long> sums;
+  WState* h_state = nullptr;
+  WBatch* h_batch = nullptr;
+  std::map<int, hipGraphExec_t> graphs;  // by L
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+void wms_workspace_free(WmsWorkspace* ws) {
+  if (!ws) return;
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (ws->h_state) (void)hipHostFree(ws->h_state);
+  if (ws->h_batch) (void)hipHostFree(ws->h_batch);
+  for (hipEvent_t e : {ws->ev0, ws->ev1})
+    if (e) (void)hipEventDestroy(e);
+  delete ws;
+}
+
+namespace {
+
+int ensure_wms(dawn_graph* g, int L) {
+  if (!g->wms) g->wms = new WmsWorkspace();
+  WmsWorkspace* ws = g->wms;
+  if (ws->n == g->n && ws->lanes == L && ws->dist.p) return DAWN_OK;
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  ws->graphs.clear();
+  ws->dist.release();
+  ws->rows.release();
+  ws->n = g->n;
+  ws->lanes = L;
+  DAWN_TRY(ws->dist.alloc((size_t)g->n * L));
+  DAWN_TRY(ws->act.alloc((size_t)2 * g->n));
+  DAWN_TRY(ws->q.alloc((size_t)2 * g->n));
+  DAWN_TRY(ws->st.alloc(1));
+  DAWN_TRY(ws->batch.alloc(1));
+  DAWN_TRY(ws->sums.alloc(64));
+  if (!ws->h_state) DAWN_HIP(hipHostMalloc((void**)&ws->h_state, sizeof(WState)));
+  if (!ws->h_batch) DAWN_HIP(hipHostMalloc((void**)&ws->h_batch, sizeof(WBatch)));
+  for (hipEvent_t* e : {&ws->ev0, &ws->ev1})
+    if (!*e) DAWN_HIP(hipEventCreate(e));
+  return DAWN_OK;
+}
+
+template <int L>
+int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+          dawn_stats* stats) {
+  DAWN_TRY(ensure_wms(g, L));
+  WmsWorkspace* ws = g->wms;
+  hipStream_t s = g->stream;
+  const bool dist_dev = dist && is_device_pointer(dist);
+  if (dist && !dist_dev) DAWN_TRY(ws->rows.alloc((size_t)L * g->n));
+  WArgs a{};
+  a.n = g->n;
+  a.row_ptr = g->row_ptr;
+  a.col = g->col;
+  a.w = g->weight;
+  a.dist = ws->dist.p;
+  a.act = ws->act.p;
+  a.q = ws->q.p;
+  a.st = ws->st.p;
+  a.batch = ws->batch.p;
+  const unsigned fill_grid = grid_for((int64_t)g->n * L);
+  const unsigned step_grid = 2048;  // 8 workgroups per CU, waves stride the queue
+  const unsigned fin_grid = grid_for(std::max<int32_t>(g->n, 1), kBlock, 2048);
+
+  hipGraphExec_t exec = nullptr;
+  auto it = ws->graphs.find(L);
+  if (it != ws->graphs.end()) {
+    exec = it->second;
+  } else {
+    hipGraph_t graph;
+    DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < kWChunk; i++) {
+      w_step_kernel<L><<<step_grid, kBlock, 0, s>>>(a);
+      w_advance_kernel<<<1, 1, 0, s>>>(a.st);
+    }
+    DAWN_HIP(hipStreamEndCapture(s, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DAWN_HIP(e);
+    ws->graphs[L] = exec;
+  }
+
+  float total_ms = 0.f;
+  std::vector<unsigned long long> h_sums(64);
+  for (int32_t b0 = 0; b0 < S; b0 += L) {
+    const int32_t lanes = std::min<int32_t>(L, S - b0);
+    WBatch& hb = *ws->h_batch;
+    std::memset(&hb, 0, sizeof hb);
+    hb.nlanes = lanes;
+    for (int i = 0; i < lanes; i++) {
+      int64_t v = sources[b0 + i] % (int64_t)g->n;
+      if (v < 0) v += g->n;
+      hb.src[i] = (int32_t)v;
+      if (stats && g->h_row_ptr[v] == g->h_row_ptr[v + 1]) stats->skipped_sources += 1;
+    }
+    DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(WBatch), hipMemcpyHostToDevice, s));
+    DAWN_HIP(hipEventRecord(ws->ev0, s));
+    w_fill_kernel<L><<<fill_grid, kBlock, 0, s>>>(a);
+    w_seed_kernel<L><<<1, 64, 0, s>>>(a);
+    int64_t launched = 0;
+    for (;;) {
+      DAWN_HIP(hipGraphLaunch(exec, s));
+      launched += kWChunk;
+      DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(WState), hipMemcpyDeviceToHost, s));
+      DAWN_HIP(hipStreamSynchronize(s));
+      if (ws->h_state->done) break;
+      if (launched > (int64_t)g->n + 2 * kWChunk) return DAWN_ERR_HIP;
+    }
+    int32_t* out = dist ? (dist_dev ? dist + (int64_t)b0 * g->n : ws->rows.p) : nullptr;
+    w_zero_kernel<<<1, 64, 0, s>>>(ws->sums.p, 64);
+    w_final_kernel<L><<<fin_grid, kBlock, 0, s>>>(a, out, g->n, ws->sums.p);
+    DAWN_HIP(hipGetLastError());
+    DAWN_HIP(hipEventRecord(ws->ev1, s));
+    DAWN_HIP(hipMemcpyAsync(h_sums.data(), ws->sums.p, sizeof(unsigned long long) * 64,
+                            hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    DAWN_HIP(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+    total_ms += ms;
+    if (dist && !dist_dev)
+      DAWN_HIP(hipMemcpy(dist + (int64_t)b0 * g->n, ws->rows.p, sizeof(int32_t) * (size_t)lanes * g->n,
+                         hipMemcpyDeviceToHost));
+    for (int i = 0; i < lanes; i++)
+      if (sums) sums[b0 + i] = (int64_t)h_sums[i];
+    if (stats) {
+      stats->batches += 1;
+      stats->steps += ws->h_state->step;
+      stats->mode_steps[DAWN_MODE_PUSH] += ws->h_state->step;
+    }
+  }
+  if (stats) stats->device_ms += total_ms;
+  return DAWN_OK;
+}
+
+}  // namespace
+
+// Weighted multi-source traversal, batches of up to 64 sources.
+int run_wmsbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+               dawn_stats* stats) {
+  if (!g || !g->weighted || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
+  if (S == 0) return DAWN_OK;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  const int32_t per = std::min<int32_t>(S, 64);
+  if (per <= 8) return run_w<8>(g, sources, S, dist, sums, stats);
+  if (per <= 16) return run_w<16>(g, sources, S, dist, sums, stats);
+  if (per <= 32) return run_w<32>(g, sources, S, dist, sums, stats);
+  return run_w<64>(g, sources, S, dist, sums, stats);
+}
+
+}  // namespace dawn

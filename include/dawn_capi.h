@@ -210,6 +210,19 @@ int dawn_mssp_profile(const dawn_graph* g, const int32_t* sources,
  * bandwidth next to the 8 TB/s peak. */
 int dawn_probe_copy(int64_t bytes, int32_t iters, double* gbps);
 
+/* Device memory for callers without a GPU framework of their own (the
+ * reference keeps its device arrays in thrust::device_vector,
+ * include/dawn/algorithm/gpu/bfs.cuh:17-18): allocation on the current
+ * device, and synchronous copies / fills (hipMemcpyDefault: any direction).
+ * Pointers from another HIP runtime loaded in the same process (a
+ * framework that bundles its own) are not valid here.
+ * dawn_graph_synchronize waits for the work queued on g's stream. */
+int dawn_device_malloc(int64_t bytes, void** out);
+int dawn_device_free(void* p);
+int dawn_memcpy(void* dst, const void* src, int64_t bytes);
+int dawn_memset(void* p, int32_t byte_value, int64_t bytes);
+int dawn_graph_synchronize(const dawn_graph* g);
+
 /* ----------------------------------------------------------- scheduler --- */
 
 /* Number of per-step features read by the scheduler MLP. */

@@ -226,3 +226,44 @@ def test_lane_words_exact(orc, words):
         assert st["skipped_sources"] == int(skipped.sum())
     _, lv = g.mssp_profile(srcs)
     assert len(lv) == -(-300 // (64 * words)) and (lv > 0).all()
+
+
+@pytest.mark.parametrize("count", [1, 9, 17, 33, 70])
+def test_weighted_batched_mssp_exact(orc, count):
+    """Batched min-plus (8, 16, 32, 64 lanes; a partial second batch at 70):
+    rows and sums equal the oracle's, with a duplicate source."""
+    log(f"case: grid 40 + 2% seed 9 weighted, {count} sources")
+    h = dawn.HostGraph.grid(40, 0.02, 9, weighted=True)
+    g = dawn.Graph(h)
+    srcs = dawn.gen_sources(h.n, count, 9)
+    if count > 5:
+        srcs[5] = srcs[2]
+    odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, h.weight, srcs)
+    dist, sums, st = g.mssp(srcs, stats=True)
+    assert np.array_equal(dist, odist)
+    assert np.array_equal(sums, osums)
+    assert st["batches"] == -(-count // 64)
+
+
+def test_weighted_batched_multigraph_and_isolated(orc):
+    """Duplicate edges with different weights keep the reference's seeding
+    (a source's later listed duplicate wins, sssp.cpp:40-46); isolated
+    sources give INF rows with 0 at the source and a zero sum."""
+    log("case: weighted multigraph n=300 with isolated vertices")
+    rng = np.random.default_rng(11)
+    n = 300
+    u = rng.integers(0, 250, 900).astype(np.int32)  # 250..299 isolated
+    v = rng.integers(0, 250, 900).astype(np.int32)
+    w = rng.integers(1, 50, 900).astype(np.int32)
+    u = np.concatenate([u, u[:200]])  # duplicates with other weights
+    v = np.concatenate([v, v[:200]])
+    w = np.concatenate([w, rng.integers(1, 50, 200).astype(np.int32)])
+    h = dawn.HostGraph.from_edges(n, u, v, w, symmetrize=True, dedup=False)
+    g = dawn.Graph(h)
+    deg = np.diff(h.row_ptr)
+    srcs = np.array([0, 1, 2, 260, 3, 299, int(np.argmax(deg))], np.int32)
+    odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, h.weight, srcs)
+    dist, sums, st = g.mssp(srcs, stats=True)
+    assert np.array_equal(dist, odist)
+    assert np.array_equal(sums, osums)
+    assert st["skipped_sources"] == int(skipped.sum()) == 2
