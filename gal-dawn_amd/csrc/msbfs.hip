@@ -257,52 +257,72 @@ __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
   }
 }
 
-// Level 0: the batch's sources (duplicates share one frontier vertex).
+// Level 0: the batch's sources (duplicates share one frontier vertex), one
+// thread per lane.  The first lane of each source vertex clears its level
+// record (the slot may hold words of an earlier batch) and marks it in the
+// level-0 bitmap; then every lane sets its bit.
 template <int W>
-__global__ void ms_seed_kernel(MsArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kMaxLanes) ms_seed_kernel(MsArgs a) {
+  __shared__ int32_t s_src[kMaxLanes];
+  __shared__ unsigned long long s_c[4];  // fv, fe, source vertices complete, their in-degree
+  __shared__ unsigned long long s_alive[kMaxWords];
   const MsBatch& b = *a.batch;
-  LevelCounters c{};
-  for (int l = 0; l < b.nlanes; l++) {
-    const int32_t v = b.src[l];
-    const int i = l >> 6;
-    const uint64_t bit = 1ull << (l & 63);
-    if (!((a.fbits[v >> 5] >> (v & 31)) & 1u)) {  // first lane at v
-      for (int k = 0; k < W; k++) a.hist[(int64_t)v * W + k] = 0ull;
-      const int d = out_deg(a, v);
-      a.fbits[v >> 5] |= 1u << (v & 31);
-      c.fv += 1;
-      c.fe += (unsigned)d;
-      c.scout += (unsigned)d;
-    }
-    a.hist[(int64_t)v * W + i] |= bit;
-    a.seen[(int64_t)v * W + i] |= bit;
-    c.flanes += 1;
-    c.alive[i] |= bit;
+  const int nl = b.nlanes;
+  const int l = threadIdx.x;
+  if (l < nl) s_src[l] = b.src[l];
+  if (l < 4) s_c[l] = 0;
+  if (l < kMaxWords) s_alive[l] = 0;
+  __syncthreads();
+  int32_t v = -1;
+  bool first = false;
+  if (l < nl) {
+    v = s_src[l];
+    first = true;
+    for (int k = 0; k < l && first; k++) first = s_src[k] != v;
   }
+  if (first) {
+    for (int k = 0; k < W; k++) a.hist[(int64_t)v * W + k] = 0ull;
+    atomicOr(&a.fbits[v >> 5], 1u << (v & 31));
+    atomicAdd(&s_c[0], 1ull);
+    atomicAdd(&s_c[1], (unsigned long long)out_deg(a, v));
+  }
+  __syncthreads();
+  if (l < nl) {
+    const uint64_t bit = 1ull << (l & 63);
+    atomicOr((unsigned long long*)&a.hist[(int64_t)v * W + (l >> 6)], (unsigned long long)bit);
+    atomicOr((unsigned long long*)&a.seen[(int64_t)v * W + (l >> 6)], (unsigned long long)bit);
+    atomicOr(&s_alive[l >> 6], (unsigned long long)bit);
+  }
+  __syncthreads();
   // Vertices still missing a lane after level 0, and their in-degree sum:
-  // a source vertex counts once (at its first lane) if every lane has it.
-  c.uv = (unsigned long long)a.n;
-  c.ue = (unsigned long long)a.m;
-  for (int l = 0; l < b.nlanes; l++) {
-    const int32_t v = b.src[l];
-    bool first = true;
-    for (int k = 0; k < l; k++) first = first && b.src[k] != v;
+  // a source vertex leaves the count (once) if every lane has it.
+  if (first) {
     bool full = true;
     for (int k = 0; k < W; k++) full = full && a.seen[(int64_t)v * W + k] == ~0ull;
-    if (first && full) {
-      c.uv -= 1;
-      c.ue -= (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+    if (full) {
+      atomicAdd(&s_c[2], 1ull);
+      atomicAdd(&s_c[3], (unsigned long long)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]));
     }
   }
-  a.ring[0] = c;
-  SchedState st{};
-  st.step = 0;
-  st.e2c = a.m;
-  st.scout = (long long)c.scout;
-  st.force_mode = -1;
-  st.last_mode = DAWN_MODE_PUSH;
-  *a.st = st;
+  __syncthreads();
+  if (l == 0) {
+    LevelCounters c{};
+    c.fv = s_c[0];
+    c.fe = s_c[1];
+    c.scout = s_c[1];
+    c.flanes = (unsigned long long)nl;
+    for (int k = 0; k < W; k++) c.alive[k] = s_alive[k];
+    c.uv = (unsigned long long)a.n - s_c[2];
+    c.ue = (unsigned long long)a.m - s_c[3];
+    a.ring[0] = c;
+    SchedState st{};
+    st.step = 0;
+    st.e2c = a.m;
+    st.scout = (long long)c.scout;
+    st.force_mode = -1;
+    st.last_mode = DAWN_MODE_PUSH;
+    *a.st = st;
+  }
 }
 
 __global__ void ms_decide_kernel(MsArgs a, SchedParams p, SchedTrace tr) {
@@ -817,29 +837,40 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
     }
     if (dist && v0 < a.n) {
       const bool full = vec_ok && v0 + 3 < a.n;
-      for (int b = 0; b < rows; b++) {
-        int32_t d[4];
-        bool wr[4];
+      // Rows 0-31 of the word from the low halves, 32-63 from the high
+      // halves: one bit-field extract per (vertex, slice).  Slices k >= K
+      // are zero, so all kSlices are summed without a test.
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        uint32_t Sh[4][kSlices], Fh[4], Vh[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          int32_t x = lo;
+          Fh[j] = (uint32_t)(F[j] >> (32 * h));
+          Vh[j] = (uint32_t)(sv[j] >> (32 * h));
 #pragma unroll
-          for (int k = 0; k < kSlices; k++)
-            if (k < K) x += (int32_t)((S[j][k] >> b) & 1ull) << k;
-          wr[j] = true;
-          if (!((F[j] >> b) & 1ull)) {
-            x = kInf;
-            wr[j] = fin && !((sv[j] >> b) & 1ull);
-          }
-          d[j] = x;
+          for (int k = 0; k < kSlices; k++) Sh[j][k] = (uint32_t)(S[j][k] >> (32 * h));
         }
-        int32_t* row = dist + (int64_t)(64 * wi + b) * stride + v0;
-        if (full) {
-          *reinterpret_cast<int4*>(row) = make_int4(d[0], d[1], d[2], d[3]);
-        } else {
+        const int bmax = min(32, rows - 32 * h);
+        for (int bb = 0; bb < bmax; bb++) {
+          int32_t d[4];
+          bool wr[4];
 #pragma unroll
-          for (int j = 0; j < 4; j++)
-            if (v0 + j < a.n && wr[j]) row[j] = d[j];
+          for (int j = 0; j < 4; j++) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int k = 0; k < kSlices; k++) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
+            const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
+            d[j] = in ? lo + (int32_t)x : kInf;
+            wr[j] = in || (fin && __builtin_amdgcn_ubfe(Vh[j], bb, 1) == 0);
+          }
+          int32_t* row = dist + (int64_t)(64 * wi + 32 * h + bb) * stride + v0;
+          if (full) {
+            *reinterpret_cast<int4*>(row) = make_int4(d[0], d[1], d[2], d[3]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+              if (v0 + j < a.n && wr[j]) row[j] = d[j];
+          }
         }
       }
     }
@@ -1064,7 +1095,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     hipGraph_t graph;
     DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o);
-    ms_seed_kernel<W><<<1, 64, 0, s>>>(a);
+    ms_seed_kernel<W><<<1, kMaxLanes, 0, s>>>(a);
     for (int i = 0; i < kChunk; i++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
       launch_step<W>(a, grid, fin_grid, s);
@@ -1190,7 +1221,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
     stage_batch(ws, sources + b0, lanes);
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o);
-    ms_seed_kernel<W><<<1, 64, 0, s>>>(a);
+    ms_seed_kernel<W><<<1, kMaxLanes, 0, s>>>(a);
     for (int step = 1; rows < max_rows && step < kHist; step++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
@@ -1283,7 +1314,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
     ws->h_batch->stride = g->n;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     DAWN_TRY(timed(0, [&] { ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o); }));
-    DAWN_TRY(timed(1, [&] { ms_seed_kernel<W><<<1, 64, 0, s>>>(a); }));
+    DAWN_TRY(timed(1, [&] { ms_seed_kernel<W><<<1, kMaxLanes, 0, s>>>(a); }));
     int lo = 0;
     for (int64_t guard_steps = 0;; guard_steps++) {
       if (guard_steps > (int64_t)g->n + 2) return DAWN_ERR_HIP;
