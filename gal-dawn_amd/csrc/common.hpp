@@ -254,6 +254,21 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x) {
   return x;
 }
 
+// In-register 32x32 bit transpose: afterwards bit r of A[c] is bit c of the
+// input A[r] (recursive block swaps, 5 rounds of 16 masked exchanges).
+__host__ __device__ __forceinline__ void transpose32(uint32_t (&A)[32]) {
+  uint32_t m = 0x0000FFFFu;
+#pragma unroll
+  for (int j = 16; j != 0; j >>= 1, m ^= (m << j)) {
+#pragma unroll
+    for (int k = 0; k < 32; k = ((k | j) + 1) & ~j) {
+      const uint32_t t = ((A[k] >> j) ^ A[k | j]) & m;
+      A[k] ^= t << j;
+      A[k | j] ^= t;
+    }
+  }
+}
+
 // Exclusive scan of one int per thread over a kBlock-thread workgroup.
 // scratch: kBlock/kWave ints of LDS.  Returns the exclusive prefix; *total
 // receives the block sum.

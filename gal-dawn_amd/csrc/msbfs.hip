@@ -59,7 +59,7 @@ constexpr int kMaxWords = 4;
 constexpr int kMaxLanes = 64 * kMaxWords;
 constexpr int kHist = 32;        // level slots
 constexpr int kChunk = 24;       // steps per captured chunk (< kHist)
-constexpr int kSlices = 6;       // bits of a relative level (2^6 > kHist)
+constexpr int kSlices = 5;       // bits of a relative level (2^5 >= kHist)
 
 struct MsBatch {
   int32_t nlanes;
@@ -98,7 +98,6 @@ struct MsArgs {
   int64_t num_oc;
   int32_t heavy;          // degree from which a vertex's edges are chunked
   int32_t variant;        // light pull variant (TuneKnobs::pull_variant)
-  int32_t deg_bits;       // bits of the largest out-degree
   LevelCounters* ring;
   SchedState* st;
   MsBatch* batch;
@@ -858,7 +857,8 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
           for (int j = 0; j < 4; j++) {
             uint32_t x = 0;
 #pragma unroll
-            for (int k = 0; k < kSlices; k++) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
+            for (int k = 0; k < kSlices; k++)
+              if (k < K) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
             const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
             d[j] = in ? lo + (int32_t)x : kInf;
             wr[j] = in || (fin && __builtin_amdgcn_ubfe(Vh[j], bb, 1) == 0);
@@ -899,7 +899,10 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       int dg[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) dg[j] = v0 + j < a.n ? out_deg(a, (int32_t)(v0 + j)) : 0;
-      for (int t = 0; t < a.deg_bits; t++) {
+      // Degree bits of this wave's vertices only (hubs are rare).
+      const int dmax = wave_max_i32(max(max(dg[0], dg[1]), max(dg[2], dg[3])));
+      const int dbits = dmax > 0 ? 32 - __builtin_clz((unsigned)dmax) : 0;
+      for (int t = 0; t < dbits; t++) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           const uint64_t dm = __ballot((dg[j] >> t) & 1);
@@ -998,7 +1001,6 @@ MsArgs make_args(dawn_graph* g) {
   a.num_oc = g->num_oc;
   a.heavy = g->knobs.heavy_degree;
   a.variant = g->knobs.pull_variant;
-  a.deg_bits = 32 - __builtin_clz((unsigned)std::max(1, g->max_out_degree));
   a.ring = ws->ring.p;
   a.st = ws->st.p;
   a.batch = ws->batch.p;
