@@ -1,5 +1,5 @@
 // Bit-packed unweighted DAWN over 64·W sources at once (batched BOVM) on one
-// HIP device, W = 1, 2 or 4 lane words per vertex.
+// HIP device, W = 1, 2, 4 or 8 lane words per vertex.
 //
 // Replaces the reference's per-source loops DAWN::MSSP_GPU::run
 // (src/algorithm/gpu/mssp.cu:9-51), DAWN::APSP_GPU::run
@@ -8,8 +8,8 @@
 // once; each vertex carries W uint64 words, bit b of word i = source lane
 // 64·i + b.  A vertex's W words are contiguous, so every random access of
 // a step (one frontier gather, one seen probe, one atomic) moves the lanes
-// of all W words in one cache line: the dense steps cost about the same
-// for 256 sources as for 64.
+// of all W words in one or two cache lines: the dense steps, bound by
+// random accesses, cost far less per source at W = 8 than at W = 1.
 //
 // Layout in HBM (per graph, reused across batches; v-major, W words each):
 //   seen   u64[npad][W]        lanes that have reached v
@@ -55,7 +55,7 @@
 
 namespace dawn {
 
-constexpr int kMaxWords = 4;
+constexpr int kMaxWords = 8;
 constexpr int kMaxLanes = 64 * kMaxWords;
 constexpr int kHist = 32;        // level slots
 constexpr int kChunk = 24;       // steps per captured chunk (< kHist)
@@ -493,12 +493,13 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     // frontier words are gathered only for in-neighbours that are in it.
     const uint32_t* fb = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
-    if (a.variant == 1) {
+    if (W <= 4 && a.variant == 1) {
       // Light vertices, edge-balanced: the workgroup's 256 vertices' in-edge
       // lists are concatenated (degree scan) and read one edge per thread,
       // coalesced; hits are OR-ed into the owner's LDS words.  Gathers stop
       // for an owner once it has found all its missing lanes.
-      __shared__ unsigned long long s_acc[kBlock * W];
+      // (W = 8 falls back to variant 0: this array would halve occupancy.)
+      __shared__ unsigned long long s_acc[kBlock * (W <= 4 ? W : 1)];
       for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += wstride) {
         const int32_t v = (int32_t)(b0 + threadIdx.x);
         uint64_t miss[W] = {};
@@ -557,11 +558,12 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
         }
       }
     } else if (a.variant == 0) {
-      pull_light_lanes<W, 4, false>(a, prev, fb, alive);
+      // Gathers in flight per lane: fewer for wide records (registers).
+      pull_light_lanes<W, (W > 4 ? 2 : 4), false>(a, prev, fb, alive);
     } else if (a.variant == 2) {
-      pull_light_lanes<W, 8, false>(a, prev, fb, alive);
+      pull_light_lanes<W, (W > 4 ? 4 : 8), false>(a, prev, fb, alive);
     } else {
-      pull_light_lanes<W, 8, true>(a, prev, fb, alive);
+      pull_light_lanes<W, (W > 4 ? 4 : 8), true>(a, prev, fb, alive);
     }
     // Heavy vertices: their in-edge lists are cut into chunks of at most
     // knobs.pull_chunk edges, one wave per chunk, merged with atomicOr.
@@ -600,14 +602,16 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
 }
 
 // Folds next[] into the level slot, seen, the frontier bitmap, the done
-// bitmap and the counters.  Each thread takes four consecutive words (4/W
-// vertices) with two 16-byte loads per array; npad is a multiple of 64, so
-// a thread's vertices never straddle it.  Threads whose vertices are all
-// done read one bitmap word and nothing else.
+// bitmap and the counters.  Each thread takes four consecutive words with
+// two 16-byte loads per array: 4/W whole vertices (W <= 4), or a quarter
+// of one vertex's record (W = 8: two threads per vertex, combined with a
+// lane shuffle).  npad is a multiple of 64, so a thread's vertices never
+// straddle it.  Vertices already done are skipped after one bitmap read.
 template <int W>
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
-  constexpr int V = 4 / W;     // vertices per thread
-  constexpr int LPW = 32 / V;  // lanes per 32-bit bitmap word
+  constexpr int TPV = W > 4 ? W / 4 : 1;       // threads per vertex
+  constexpr int V = W < 4 ? 4 / W : 1;         // vertices per thread
+  constexpr int LPW = 32 * TPV / V;            // lanes per 32-bit bitmap word
   constexpr unsigned kAll = (1u << V) - 1;
   const SchedState st = *a.st;
   if (st.done) return;
@@ -615,38 +619,76 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
   uint32_t* fb = a.fbits + (int64_t)(lv % a.H) * a.fwords;
   LevelCounters* c_out = a.ring + (lv % kRing);
-  uint64_t alive_prev[W];
-#pragma unroll
-  for (int k = 0; k < W; k++) alive_prev[k] = a.ring[(lv - 1) % kRing].alive[k];
   const int lane = lane_id();
+  const int sub = lane % TPV;  // quarter of the record (W = 8)
+  // Live lanes of the previous level for this thread's words (word i of a
+  // vertex record for W <= 4; words 4*sub+i for W = 8).
+  uint64_t alive_prev[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const unsigned long long* al = a.ring[(lv - 1) % kRing].alive;
+    alive_prev[i] = W > 4 ? (sub ? al[4 + i] : al[i]) : (i < W ? al[i] : 0ull);
+  }
   MsAcc<W> acc;
-  constexpr int64_t kTile = (int64_t)kBlock * V;
+  constexpr int64_t kTile = (int64_t)kBlock / TPV * V;
   for (int64_t b0 = (int64_t)blockIdx.x * kTile; b0 < a.npad; b0 += (int64_t)gridDim.x * kTile) {
-    const int64_t v0 = b0 + (int64_t)threadIdx.x * V;
-    unsigned nib = 0, dnib = kAll;
+    const int64_t v0 = b0 + (int64_t)(threadIdx.x / TPV) * V;
+    unsigned dnib = kAll;
     if (v0 < a.npad) dnib = (a.done[v0 >> 5] >> (v0 & 31)) & kAll;
-    if (v0 < a.npad && dnib != kAll) {
-      // A done vertex has next == 0 (no step writes lanes it already has)
-      // and is complete, so the code below leaves it unchanged.
-      const int64_t w0 = v0 * W;
-      ulonglong2* np2 = reinterpret_cast<ulonglong2*>(a.next + w0);
-      ulonglong2* sp2 = reinterpret_cast<ulonglong2*>(a.seen + w0);
-      ulonglong2* cp2 = reinterpret_cast<ulonglong2*>(cur + w0);
+    const bool work = v0 < a.npad && dnib != kAll;  // same for the threads of a vertex
+    const int64_t w0 = v0 * W + 4 * sub;
+    ulonglong2* np2 = reinterpret_cast<ulonglong2*>(a.next + w0);
+    ulonglong2* sp2 = reinterpret_cast<ulonglong2*>(a.seen + w0);
+    ulonglong2* cp2 = reinterpret_cast<ulonglong2*>(cur + w0);
+    uint64_t nx[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, nw[4] = {0, 0, 0, 0};
+    if (work) {
       const ulonglong2 n01 = np2[0], n23 = np2[1];
       const ulonglong2 s01 = sp2[0], s23 = sp2[1];
-      const uint64_t nx[4] = {n01.x, n01.y, n23.x, n23.y};
-      const uint64_t sv[4] = {s01.x, s01.y, s23.x, s23.y};
-      uint64_t nw[4];
+      nx[0] = n01.x; nx[1] = n01.y; nx[2] = n23.x; nx[3] = n23.y;
+      sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
 #pragma unroll
       for (int k = 0; k < 4; k++) nw[k] = nx[k] & ~sv[k];
+    }
+    // Per vertex: arrivals, missing live lanes, popcount (this thread's part).
+    uint64_t any[V], unv[V];
+    unsigned pc[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      any[j] = 0;
+      unv[j] = 0;
+      pc[j] = 0;
+      constexpr int WPV = W > 4 ? 4 : W;  // this thread's words of one vertex
+#pragma unroll
+      for (int i = 0; i < WPV; i++) {
+        const uint64_t x = nw[j * WPV + i];
+        any[j] |= x;
+        pc[j] += (unsigned)__popcll(x);
+        unv[j] |= ~(sv[j * WPV + i] | x) & alive_prev[i];
+        if constexpr (W > 4) {
+          acc.alive[i] |= sub ? 0ull : x;
+          acc.alive[4 + i] |= sub ? x : 0ull;
+        } else {
+          acc.alive[i] |= x;
+        }
+      }
+    }
+    if constexpr (TPV > 1) {
+#pragma unroll
+      for (int o = 1; o < TPV; o <<= 1) {
+        any[0] |= shfl_xor_u64(any[0], o);
+        unv[0] |= shfl_xor_u64(unv[0], o);
+        pc[0] += __shfl_xor(pc[0], o);
+      }
+    }
+    unsigned nib = 0;
+    if (work) {
       if (nx[0] | nx[1]) np2[0] = make_ulonglong2(0ull, 0ull);
       if (nx[2] | nx[3]) np2[1] = make_ulonglong2(0ull, 0ull);
-      // Level words only where something arrived (the frontier bitmap
-      // marks which entries of the slot are valid), but always a vertex's
-      // whole record: with W = 4 both halves belong to one vertex.
-      const bool any4 = (nw[0] | nw[1] | nw[2] | nw[3]) != 0;
-      const bool half0 = W == 4 ? any4 : (nw[0] | nw[1]) != 0;
-      const bool half1 = W == 4 ? any4 : (nw[2] | nw[3]) != 0;
+      // Level words only where something arrived (the frontier bitmap marks
+      // which entries of the slot are valid), but always a vertex's whole
+      // record.
+      const bool half0 = W >= 4 ? any[0] != 0 : (nw[0] | nw[1]) != 0;
+      const bool half1 = W >= 4 ? any[0] != 0 : (nw[2] | nw[3]) != 0;
       if (half0) {
         cp2[0] = make_ulonglong2(nw[0], nw[1]);
         sp2[0] = make_ulonglong2(sv[0] | nw[0], sv[1] | nw[1]);
@@ -657,38 +699,32 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       }
       // Padding vertices (v >= n) have seen = ~0 and next = 0: never new,
       // never unvisited.
+      if (sub == 0) {
 #pragma unroll
-      for (int j = 0; j < V; j++) {
-        const int32_t v = (int32_t)(v0 + j);
-        uint64_t any = 0, unv = 0;
-        unsigned pc = 0;
-#pragma unroll
-        for (int i = 0; i < W; i++) {
-          const uint64_t x = nw[j * W + i];
-          any |= x;
-          pc += (unsigned)__popcll(x);
-          acc.alive[i] |= x;
-          unv |= ~(sv[j * W + i] | x) & alive_prev[i];
-        }
-        if (any) {
-          nib |= 1u << j;
-          const int d = out_deg(a, v);
-          acc.fv += 1;
-          acc.fe += (unsigned)d;
-          acc.scout += (unsigned)(d > 0 ? d : 1);
-          acc.lanes += pc;
-        }
-        if (unv) {  // still misses a live lane
-          acc.uv += 1;
-          acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
-        } else {
-          dnib |= 1u << j;
+        for (int j = 0; j < V; j++) {
+          const int32_t v = (int32_t)(v0 + j);
+          if (any[j]) {
+            nib |= 1u << j;
+            const int d = out_deg(a, v);
+            acc.fv += 1;
+            acc.fe += (unsigned)d;
+            acc.scout += (unsigned)(d > 0 ? d : 1);
+            acc.lanes += pc[j];
+          }
+          if (unv[j]) {  // still misses a live lane
+            acc.uv += 1;
+            acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+          } else {
+            dnib |= 1u << j;
+          }
         }
       }
     }
+    if (sub != 0) dnib = 0;  // the first thread of a vertex reports its bits
     // Frontier and done bitmaps: LPW lanes (32 vertices) per word.
-    unsigned x = nib << (V * (lane % LPW));
-    unsigned y = dnib << (V * (lane % LPW));
+    const int pos = ((lane / TPV) * V) & 31;
+    unsigned x = nib << pos;
+    unsigned y = dnib << pos;
 #pragma unroll
     for (int o = 1; o < LPW; o <<= 1) {
       x |= __shfl_xor(x, o);
@@ -741,7 +777,8 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
 template <int W>
 __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int from_state,
                                                           int lo_arg, int hi_arg, int final_arg) {
-  constexpr int TPB = 4 / W;  // vertex tiles per workgroup
+  constexpr int TPB = W < 4 ? 4 / W : 1;  // vertex tiles per workgroup
+  constexpr int BPT = W > 4 ? W / 4 : 1;  // workgroups per tile (grid is a multiple)
   __shared__ unsigned long long s_sum[64 * W], s_reach[64 * W], s_edges[64 * W];
   int lo = lo_arg, hi = hi_arg, fin = final_arg;
   if (from_state) {
@@ -754,10 +791,10 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   const int nl = a.batch->nlanes;
   int32_t* const dist = a.batch->dist;
   const int64_t stride = a.batch->stride;
-  if (threadIdx.x < 64 * W) {
-    s_sum[threadIdx.x] = 0;
-    s_reach[threadIdx.x] = 0;
-    s_edges[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < 64 * W; i += kBlock) {
+    s_sum[i] = 0;
+    s_reach[i] = 0;
+    s_edges[i] = 0;
   }
   __syncthreads();
   // Levels are stored relative to lo in K bit slices: S_k = OR of the
@@ -766,14 +803,17 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   while ((1 << K) < hi - lo) K++;
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  const int wi = wv % W;     // lane word of this wave
-  const int tile = wv / W;   // tile of this wave within the workgroup
+  // Lane word of this wave, and its tile: W <= 4: the workgroup's four
+  // waves cover 4/W tiles; W = 8: two workgroups share a tile.
+  const int wi = W > 4 ? (int)(blockIdx.x % BPT) * 4 + wv : wv % W;
+  const int tile = W > 4 ? 0 : wv / W;
+  const int64_t tile0 = W > 4 ? (int64_t)(blockIdx.x / BPT) : (int64_t)blockIdx.x * TPB + tile;
   const int rows = min(64, nl - 64 * wi);  // lanes of this word in the batch
   // With one flush of every level (lo == 0, final) every (lane, vertex) is
   // written: level or INF.
   const bool vec_ok = dist && (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0) && lo == 0 && fin;
-  const int64_t wstride = (int64_t)gridDim.x * TPB * 256;
-  for (int64_t base = ((int64_t)blockIdx.x * TPB + tile) * 256; base < a.n; base += wstride) {
+  const int64_t wstride = (int64_t)(gridDim.x / BPT) * TPB * 256;
+  for (int64_t base = tile0 * 256; base < a.n; base += wstride) {
     const int64_t v0 = base + 4 * lane;
     uint64_t F[4], S[4][kSlices], sv[4];
 #pragma unroll
@@ -1032,9 +1072,12 @@ unsigned ms_grid(const dawn_graph* g) {
 unsigned ms_fin_grid(const dawn_graph* g, int W) {
   return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, 2048u);
 }
-// Distance pass: 1024/W vertices per workgroup.
+// Distance pass: 1024/W vertices per workgroup (W = 8: two workgroups per
+// 256-vertex tile, so the grid is even).
 unsigned ms_final_grid(const dawn_graph* g, int W) {
-  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, (unsigned)g->knobs.dense_blocks);
+  unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, (unsigned)g->knobs.dense_blocks);
+  if (W > 4) grid = std::max(2u, grid & ~1u);
+  return grid;
 }
 
 // Stages one batch's sources (taken mod n) into the pinned batch block.
@@ -1354,7 +1397,8 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, i
   switch (words_for(g, S)) {
     case 1: return run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats);
     case 2: return run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats);
-    default: return run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats);
+    case 4: return run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats);
+    default: return run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats);
   }
 }
 
@@ -1371,8 +1415,11 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
     case 2:
       return run_ms_record_w<2>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
                                 rows_out, policy);
-    default:
+    case 4:
       return run_ms_record_w<4>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
+                                rows_out, policy);
+    default:
+      return run_ms_record_w<8>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
                                 rows_out, policy);
   }
 }
@@ -1385,7 +1432,8 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   switch (words_for(g, S)) {
     case 1: return run_ms_profile_w<1>(g, sources, S, policy, kernel_ms, launches, levels_out);
     case 2: return run_ms_profile_w<2>(g, sources, S, policy, kernel_ms, launches, levels_out);
-    default: return run_ms_profile_w<4>(g, sources, S, policy, kernel_ms, launches, levels_out);
+    case 4: return run_ms_profile_w<4>(g, sources, S, policy, kernel_ms, launches, levels_out);
+    default: return run_ms_profile_w<8>(g, sources, S, policy, kernel_ms, launches, levels_out);
   }
 }
 

@@ -68,7 +68,8 @@ int sample_ms(dawn_graph* g, const std::vector<int32_t>& src, double* ms) {
 bool knobs_valid(const TuneKnobs& k) {
   return k.dense_blocks >= 1 && k.dense_blocks <= 65536 && k.heavy_degree >= 1 &&
          k.pull_chunk >= 1 && k.ss_chunk >= 0 && k.ss_chunk <= 1024 && k.pull_variant >= 0 &&
-         k.pull_variant <= 3 && (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4);
+         k.pull_variant <= 3 &&
+         (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4 || k.lane_words == 8);
 }
 
 int apply(dawn_graph* g, const TuneKnobs& k) {
@@ -104,7 +105,7 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
   if (!g || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   // Sample sources: the highest-degree vertex and spread vertices with edges.
   std::vector<int32_t> src;
-  const int want = g->weighted ? 2 : 256;
+  const int want = g->weighted ? 2 : 512;
   for (int32_t v = 0; v < g->n && (int)src.size() < want; v += std::max(1, g->n / (4 * want)))
     if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
   if (src.empty()) return DAWN_ERR_INVALID_ARG;
@@ -134,8 +135,8 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       DAWN_TRY(consider(k));
     }
   } else {
-    // The same 256 sample sources at every width (4, 2 or 1 batches).
-    for (int lw : {1, 2, 4}) {
+    // The same 512 sample sources at every width (8, 4, 2 or 1 batches).
+    for (int lw : {1, 2, 4, 8}) {
       TuneKnobs k = best;
       k.lane_words = lw;
       DAWN_TRY(consider(k));

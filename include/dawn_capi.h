@@ -314,7 +314,7 @@ int dawn_tune_save(const char* path);
  * dense batched kernels, degree from which edge lists are cut into wave
  * chunks, chunk length, single-source steps per captured graph (0 = auto),
  * light-pull variant (0..3), 64-lane words per vertex of the batched engine
- * (1, 2 or 4: 64, 128 or 256 sources per batch).  Knobs change time only,
+ * (1, 2, 4 or 8: 64, 128, 256 or 512 sources per batch).  Knobs change time only,
  * never results.  dawn_graph_set_knobs rejects values out of range. */
 enum {
   DAWN_KNOB_DENSE_BLOCKS = 0,

@@ -207,16 +207,16 @@ def test_tuned_knobs_keep_results_exact(orc, tmp_path):
     assert len(open(p).read().splitlines()) >= 2
 
 
-@pytest.mark.parametrize("words", [1, 2, 4])
+@pytest.mark.parametrize("words", [1, 2, 4, 8])
 def test_lane_words_exact(orc, words):
-    """64, 128 and 256 sources per batch, with a partial last batch and
+    """64, 128, 256 and 512 sources per batch, with a partial last batch and
     duplicate sources: rows and sums equal the oracle's."""
-    log(f"case: RMAT scale 13 ef 8 seed 6, 300 sources, lane_words={words}")
+    log(f"case: RMAT scale 13 ef 8 seed 6, 700 sources, lane_words={words}")
     h = dawn.HostGraph.rmat(13, 8, 6)
     g = dawn.Graph(h)
     g.set_knobs(lane_words=words)
     assert g.knobs()["lane_words"] == words
-    srcs = dawn.gen_sources(h.n, 300, 6)
+    srcs = dawn.gen_sources(h.n, 700, 6)
     srcs[7] = srcs[3]
     odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
     for p in ("fixed", "pull", "push", "bitmap"):
@@ -225,7 +225,7 @@ def test_lane_words_exact(orc, words):
         assert np.array_equal(sums, osums), p
         assert st["skipped_sources"] == int(skipped.sum())
     _, lv = g.mssp_profile(srcs)
-    assert len(lv) == -(-300 // (64 * words)) and (lv > 0).all()
+    assert len(lv) == -(-700 // (64 * words)) and (lv > 0).all()
 
 
 @pytest.mark.parametrize("count", [1, 9, 17, 33, 70])
