@@ -521,7 +521,7 @@ class Graph:
                "dawn_step")
 
     KNOB_NAMES = ("dense_blocks", "heavy_degree", "pull_chunk", "ss_chunk", "pull_variant",
-                  "lane_words")
+                  "lane_words", "hub_order")
 
     def knobs(self) -> dict:
         """The graph's launch knobs (dawn_graph_get_knobs)."""

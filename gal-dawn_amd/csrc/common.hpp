@@ -80,11 +80,16 @@ struct TuneKnobs {
   int32_t pull_variant = 0;     // light pull: 0 per lane 4 in flight, 1 edge-balanced,
                                 // 2 per lane 8 in flight, 3 as 2 with streaming column loads
   int32_t lane_words = 8;       // batched engine: 64-lane words per vertex (1, 2, 4, 8)
+  int32_t hub_order = 1;        // batched engine: hub-first vertex order (MsView)
 };
+
+// Graphs below this size keep their own order in the batched engine.
+constexpr int32_t kViewMinVertices = 1 << 16;
 
 struct MsWorkspace;   // 64-lane batched engine state (msbfs.hip)
 struct SsWorkspace;   // single-source engines state (bfs.hip / sssp.hip)
 struct WmsWorkspace;  // batched weighted engine state (wmsbfs.hip)
+struct MsView;        // hub-first order of the batched engine (view.hip)
 
 }  // namespace dawn
 
@@ -125,7 +130,29 @@ struct dawn_graph {
   dawn::MsWorkspace* ms = nullptr;
   dawn::SsWorkspace* ss = nullptr;
   dawn::WmsWorkspace* wms = nullptr;
+  dawn::MsView* view = nullptr;
 };
+
+namespace dawn {
+// Hub-first renumbering of a graph for the batched unweighted engine: the
+// n/16 vertices of largest out-degree first, the others in their order.
+// Device arrays are in the new numbering; perm maps old ids to new ones.
+struct MsView {
+  int32_t k = 0;
+  std::vector<int32_t> h_perm, h_row_ptr, h_in_row_ptr;
+  int32_t* row_ptr = nullptr;
+  int32_t* col = nullptr;
+  int32_t* in_row_ptr = nullptr;  // == row_ptr / col for undirected graphs
+  int32_t* in_col = nullptr;
+  int32_t* perm = nullptr;
+  int32_t *hc_v = nullptr, *hc_beg = nullptr, *hc_end = nullptr;
+  int64_t num_hc = 0;
+  int32_t *oc_v = nullptr, *oc_beg = nullptr, *oc_end = nullptr;
+  int64_t num_oc = 0;
+};
+int ms_view_ensure(dawn_graph* g);
+void ms_view_free(dawn_graph* g);
+}  // namespace dawn
 
 namespace dawn {
 

@@ -43,7 +43,6 @@ bool is_device_pointer(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
-namespace {
 int upload_i32(int32_t** dst, const std::vector<int32_t>& src) {
   hipError_t e = hipMalloc((void**)dst, sizeof(int32_t) * std::max<size_t>(src.size(), 1));
   if (e != hipSuccess) {
@@ -55,6 +54,7 @@ int upload_i32(int32_t** dst, const std::vector<int32_t>& src) {
   return DAWN_OK;
 }
 
+namespace {
 void free_chunks(dawn_graph* g) {
   if (g->oc_v && g->oc_v != g->hc_v) {
     (void)hipFree(g->oc_v);
@@ -69,32 +69,37 @@ void free_chunks(dawn_graph* g) {
 }
 }  // namespace
 
+// Chunk list of the rows of `rp` with at least `heavy` entries, cut into
+// pieces of at most `chunk`: (row, first, end) triples on the device.
+int build_chunk_list(const std::vector<int32_t>& rp, int heavy, int chunk, int32_t** dv,
+                     int32_t** db, int32_t** de, int64_t* count) {
+  std::vector<int32_t> hv, hb, he;
+  const int32_t n = (int32_t)rp.size() - 1;
+  for (int32_t v = 0; v < n; v++) {
+    const int32_t b = rp[v], e = rp[v + 1];
+    if (e - b < heavy) continue;
+    for (int32_t k = b; k < e; k += chunk) {
+      hv.push_back(v);
+      hb.push_back(k);
+      he.push_back(std::min(e, k + chunk));
+    }
+  }
+  *count = (int64_t)hv.size();
+  DAWN_TRY(upload_i32(dv, hv));
+  DAWN_TRY(upload_i32(db, hb));
+  return upload_i32(de, he);
+}
+
 // Heavy in-edge chunks for the pull step (dawn_graph::hc_v) and heavy
 // out-edge chunks for the dense top-down step, from g->knobs.
 int build_chunks(dawn_graph* g) {
   free_chunks(g);
   const int heavy = g->knobs.heavy_degree, chunk = g->knobs.pull_chunk;
-  auto chunks = [&](const std::vector<int32_t>& rp, int32_t** dv, int32_t** db, int32_t** de,
-                    int64_t* count) -> int {
-    std::vector<int32_t> hv, hb, he;
-    for (int32_t v = 0; v < g->n; v++) {
-      const int32_t b = rp[v], e = rp[v + 1];
-      if (e - b < heavy) continue;
-      for (int32_t k = b; k < e; k += chunk) {
-        hv.push_back(v);
-        hb.push_back(k);
-        he.push_back(std::min(e, k + chunk));
-      }
-    }
-    *count = (int64_t)hv.size();
-    DAWN_TRY(upload_i32(dv, hv));
-    DAWN_TRY(upload_i32(db, hb));
-    return upload_i32(de, he);
-  };
-  DAWN_TRY(chunks(g->directed ? g->h_in_row_ptr : g->h_row_ptr, &g->hc_v, &g->hc_beg, &g->hc_end,
-                  &g->num_hc));
+  DAWN_TRY(build_chunk_list(g->directed ? g->h_in_row_ptr : g->h_row_ptr, heavy, chunk, &g->hc_v,
+                            &g->hc_beg, &g->hc_end, &g->num_hc));
   if (g->directed) {
-    DAWN_TRY(chunks(g->h_row_ptr, &g->oc_v, &g->oc_beg, &g->oc_end, &g->num_oc));
+    DAWN_TRY(build_chunk_list(g->h_row_ptr, heavy, chunk, &g->oc_v, &g->oc_beg, &g->oc_end,
+                              &g->num_oc));
   } else {
     g->oc_v = g->hc_v;
     g->oc_beg = g->hc_beg;
@@ -208,6 +213,7 @@ int dawn_graph_destroy(dawn_graph* g) {
   ms_workspace_free(g->ms);
   ss_workspace_free(g->ss);
   wms_workspace_free(g->wms);
+  ms_view_free(g);
   if (g->directed) {
     if (g->in_row_ptr) (void)hipFree(g->in_row_ptr);
     if (g->in_col) (void)hipFree(g->in_col);

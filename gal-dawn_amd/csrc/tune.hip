@@ -10,7 +10,7 @@
 // graph's class (log2 n, log2 average degree, directed, weighted); graphs
 // created later in that class start from the cached knobs.  The cache
 // persists as text: one line "class dense_blocks heavy_degree pull_chunk
-// ss_chunk pull_variant lane_words".
+// ss_chunk pull_variant lane_words hub_order".
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,15 +69,18 @@ bool knobs_valid(const TuneKnobs& k) {
   return k.dense_blocks >= 1 && k.dense_blocks <= 65536 && k.heavy_degree >= 1 &&
          k.pull_chunk >= 1 && k.ss_chunk >= 0 && k.ss_chunk <= 1024 && k.pull_variant >= 0 &&
          k.pull_variant <= 3 &&
-         (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4 || k.lane_words == 8);
+         (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4 || k.lane_words == 8) &&
+         (k.hub_order == 0 || k.hub_order == 1);
 }
 
 int apply(dawn_graph* g, const TuneKnobs& k) {
   std::lock_guard<std::mutex> guard(g->lock);  // traversals hold it while they run
   if (g->stream) DAWN_HIP(hipStreamSynchronize(g->stream));
   const bool rechunk = k.heavy_degree != g->knobs.heavy_degree || k.pull_chunk != g->knobs.pull_chunk;
+  const bool reorder = rechunk || k.hub_order != g->knobs.hub_order;
   g->knobs = k;
   invalidate_graphs(g);
+  if (reorder) ms_view_free(g);  // rebuilt by the next batched traversal
   if (rechunk) {
     (void)hipSetDevice(g->device);
     DAWN_TRY(build_chunks(g));
@@ -119,9 +122,10 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     DAWN_TRY(sample_ms(g, src, &ms));
     if (verbose)
       std::fprintf(stderr,
-                   "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d variant=%d words=%d: %.3f ms\n",
+                   "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d variant=%d words=%d hubs=%d: "
+                   "%.3f ms\n",
                    class_key(g).c_str(), k.dense_blocks, k.heavy_degree, k.pull_chunk, k.ss_chunk,
-                   k.pull_variant, k.lane_words, ms);
+                   k.pull_variant, k.lane_words, k.hub_order, ms);
     if (ms < best_ms) {
       best_ms = ms;
       best = k;
@@ -136,6 +140,11 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     }
   } else {
     // The same 512 sample sources at every width (8, 4, 2 or 1 batches).
+    for (int ho : {0, 1}) {
+      TuneKnobs k = best;
+      k.hub_order = ho;
+      DAWN_TRY(consider(k));
+    }
     for (int lw : {1, 2, 4, 8}) {
       TuneKnobs k = best;
       k.lane_words = lw;
@@ -182,10 +191,11 @@ int dawn_tune_load(const char* path) {
   std::map<std::string, TuneKnobs> loaded;
   int rc = DAWN_OK;
   for (;;) {
-    const int got = std::fscanf(f, "%63s %d %d %d %d %d %d", key, &k.dense_blocks, &k.heavy_degree,
-                                &k.pull_chunk, &k.ss_chunk, &k.pull_variant, &k.lane_words);
+    const int got = std::fscanf(f, "%63s %d %d %d %d %d %d %d", key, &k.dense_blocks,
+                                &k.heavy_degree, &k.pull_chunk, &k.ss_chunk, &k.pull_variant,
+                                &k.lane_words, &k.hub_order);
     if (got == EOF) break;
-    if (got != 7 || !knobs_valid(k)) {
+    if (got != 8 || !knobs_valid(k)) {
       rc = DAWN_ERR_FORMAT;
       break;
     }
@@ -204,9 +214,9 @@ int dawn_tune_save(const char* path) {
   if (!f) return DAWN_ERR_IO;
   std::lock_guard<std::mutex> l(g_tune_lock);
   for (auto& kv : g_tune_cache)
-    std::fprintf(f, "%s %d %d %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
+    std::fprintf(f, "%s %d %d %d %d %d %d %d\n", kv.first.c_str(), kv.second.dense_blocks,
                  kv.second.heavy_degree, kv.second.pull_chunk, kv.second.ss_chunk,
-                 kv.second.pull_variant, kv.second.lane_words);
+                 kv.second.pull_variant, kv.second.lane_words, kv.second.hub_order);
   std::fclose(f);
   return DAWN_OK;
 }
@@ -214,8 +224,8 @@ int dawn_tune_save(const char* path) {
 int dawn_graph_get_knobs(const dawn_graph* g, int32_t* knobs) {
   if (!g || !knobs) return DAWN_ERR_INVALID_ARG;
   const TuneKnobs& k = g->knobs;
-  const int32_t v[DAWN_NUM_KNOBS] = {k.dense_blocks, k.heavy_degree, k.pull_chunk,
-                                     k.ss_chunk,     k.pull_variant, k.lane_words};
+  const int32_t v[DAWN_NUM_KNOBS] = {k.dense_blocks, k.heavy_degree, k.pull_chunk, k.ss_chunk,
+                                     k.pull_variant, k.lane_words,   k.hub_order};
   std::memcpy(knobs, v, sizeof v);
   return DAWN_OK;
 }
@@ -229,6 +239,7 @@ int dawn_graph_set_knobs(dawn_graph* g, const int32_t* knobs) {
   k.ss_chunk = knobs[DAWN_KNOB_SS_CHUNK];
   k.pull_variant = knobs[DAWN_KNOB_PULL_VARIANT];
   k.lane_words = knobs[DAWN_KNOB_LANE_WORDS];
+  k.hub_order = knobs[DAWN_KNOB_HUB_ORDER];
   if (!knobs_valid(k)) return DAWN_ERR_INVALID_ARG;
   return apply(g, k);
 }

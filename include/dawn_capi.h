@@ -305,7 +305,7 @@ int dawn_step(const dawn_graph* g, const dawn_frontier* f, int32_t mode, void* s
  * winners to g and stores them in the in-process cache, from which graphs
  * created later in the same class start; load/save persist the cache as
  * text, one line per class: "class dense_blocks heavy_degree pull_chunk
- * ss_chunk pull_variant lane_words". */
+ * ss_chunk pull_variant lane_words hub_order". */
 int dawn_tune(const dawn_graph* g, int32_t verbose);
 int dawn_tune_load(const char* path);
 int dawn_tune_save(const char* path);
@@ -314,8 +314,10 @@ int dawn_tune_save(const char* path);
  * dense batched kernels, degree from which edge lists are cut into wave
  * chunks, chunk length, single-source steps per captured graph (0 = auto),
  * light-pull variant (0..3), 64-lane words per vertex of the batched engine
- * (1, 2, 4 or 8: 64, 128, 256 or 512 sources per batch).  Knobs change time only,
- * never results.  dawn_graph_set_knobs rejects values out of range. */
+ * (1, 2, 4 or 8: 64, 128, 256 or 512 sources per batch), hub-first vertex
+ * order of the batched engine (0 or 1; graphs of 65,536 vertices or more).
+ * Knobs change time only, never results.  dawn_graph_set_knobs rejects
+ * values out of range. */
 enum {
   DAWN_KNOB_DENSE_BLOCKS = 0,
   DAWN_KNOB_HEAVY_DEGREE = 1,
@@ -323,7 +325,8 @@ enum {
   DAWN_KNOB_SS_CHUNK = 3,
   DAWN_KNOB_PULL_VARIANT = 4,
   DAWN_KNOB_LANE_WORDS = 5,
-  DAWN_NUM_KNOBS = 6
+  DAWN_KNOB_HUB_ORDER = 6,
+  DAWN_NUM_KNOBS = 7
 };
 int dawn_graph_get_knobs(const dawn_graph* g, int32_t* knobs);
 int dawn_graph_set_knobs(dawn_graph* g, const int32_t* knobs);

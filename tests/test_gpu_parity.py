@@ -267,3 +267,32 @@ def test_weighted_batched_multigraph_and_isolated(orc):
     assert np.array_equal(dist, odist)
     assert np.array_equal(sums, osums)
     assert st["skipped_sources"] == int(skipped.sum()) == 2
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_hub_order_view_exact(orc, directed):
+    """Hub-first renumbering of the batched engine (graphs >= 65,536
+    vertices): rows in the original order, identical with and without it."""
+    log(f"case: RMAT scale 16 ef 8 seed 12, directed={directed}, hub order on/off")
+    h = dawn.HostGraph.rmat(16, 8, 12)
+    if directed:
+        # Keep one direction of each edge (lower id -> higher id).
+        rows = np.repeat(np.arange(h.n, dtype=np.int32), np.diff(h.row_ptr))
+        keep = rows < h.col
+        h = dawn.HostGraph.from_csr(*_directed_csr(h.n, rows[keep], h.col[keep]), directed=True)
+    g = dawn.Graph(h)
+    srcs = dawn.gen_sources(h.n, 130, 12)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    for hub in (1, 0):
+        g.set_knobs(hub_order=hub)
+        dist, sums = g.mssp(srcs, policy="fixed")
+        assert np.array_equal(dist, odist), hub
+        assert np.array_equal(sums, osums), hub
+
+
+def _directed_csr(n, u, v):
+    order = np.lexsort((v, u))
+    u, v = u[order], v[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, u + 1, 1)
+    return np.cumsum(rp).astype(np.int32), v.astype(np.int32)

@@ -177,16 +177,16 @@ def test_no_device_calls_fail_cleanly():
 
 def test_tune_cache_roundtrip(tmp_path):
     p = tmp_path / "tune.txt"
-    p.write_text("n20_d5_UP 1024 32 512 0 1 4\nn12_d3_UW 2048 16 256 8 0 1\n")
+    p.write_text("n20_d5_UP 1024 32 512 0 1 4 1\nn12_d3_UW 2048 16 256 8 0 1 0\n")
     dawn.tune_load(str(p))
     q = tmp_path / "out.txt"
     dawn.tune_save(str(q))
     lines = set(q.read_text().splitlines())
-    assert {"n20_d5_UP 1024 32 512 0 1 4", "n12_d3_UW 2048 16 256 8 0 1"} <= lines
+    assert {"n20_d5_UP 1024 32 512 0 1 4 1", "n12_d3_UW 2048 16 256 8 0 1 0"} <= lines
     bad = tmp_path / "bad.txt"
     bad.write_text("n1 x\n")
     with pytest.raises(dawn.DawnError):
         dawn.tune_load(str(bad))
-    bad.write_text("n3_d1_UP 1024 32 512 0 1 3\n")  # 3 lane words: not a batch width
+    bad.write_text("n3_d1_UP 1024 32 512 0 1 3 1\n")  # 3 lane words: not a batch width
     with pytest.raises(dawn.DawnError):
         dawn.tune_load(str(bad))
