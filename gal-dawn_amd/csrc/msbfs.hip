@@ -98,6 +98,7 @@ struct MsArgs {
   int64_t num_oc;
   int32_t heavy;          // degree from which a vertex's edges are chunked
   int32_t variant;        // light pull variant (TuneKnobs::pull_variant)
+  const int32_t* perm;    // original id -> position (hub-first view), or null
   LevelCounters* ring;
   SchedState* st;
   MsBatch* batch;
@@ -823,7 +824,16 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
 #pragma unroll
       for (int k = 0; k < kSlices; k++) S[j][k] = 0;
     }
-    if (v0 + 3 < a.n) {
+    // Rows are in the original order; the records live at q[j], the hub-
+    // first position of v0 + j (identity without a view).
+    const bool ident = a.perm == nullptr;
+    int64_t q[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t v = v0 + j;
+      q[j] = ident || v >= a.n ? v : (int64_t)a.perm[v];
+    }
+    if (v0 + 3 < a.n && ident) {
       if constexpr (W == 1) {
         const ulonglong2* sp = reinterpret_cast<const ulonglong2*>(a.seen + v0);
         const ulonglong2 s01 = sp[0], s23 = sp[1];
@@ -858,10 +868,41 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
             if ((r >> k) & 1) S[j][k] |= w[j];
         }
       }
+    } else if (v0 + 3 < a.n) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) sv[j] = a.seen[q[j] * W + wi];
+      // Outside the hub range the four positions are consecutive: their
+      // bitmap bits come from one (or two adjacent) words.
+      const bool run = q[1] == q[0] + 1 && q[2] == q[0] + 2 && q[3] == q[0] + 3;
+      const int64_t qw = q[0] >> 5;
+      const int qb = (int)(q[0] & 31);
+      for (int l = lo; l < hi; l++) {
+        const uint32_t* fbl = a.fbits + (int64_t)(l % a.H) * a.fwords;
+        unsigned fm = 0;
+        if (run) {
+          fm = fbl[qw] >> qb;
+          if (qb > 28) fm |= fbl[qw + 1] << (32 - qb);
+          fm &= 15u;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; j++) fm |= ((fbl[q[j] >> 5] >> (q[j] & 31)) & 1u) << j;
+        }
+        if (!fm) continue;
+        const uint64_t* hb = a.hist + (int64_t)(l % a.H) * a.npad * W;
+        const int r = l - lo;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint64_t w = ((fm >> j) & 1u) ? hb[q[j] * W + wi] : 0ull;
+          F[j] |= w;
+#pragma unroll
+          for (int k = 0; k < kSlices; k++)
+            if ((r >> k) & 1) S[j][k] |= w;
+        }
+      }
     } else {
       for (int j = 0; j < 4; j++) {
-        const int64_t v = v0 + j;
-        if (v >= a.n) continue;
+        if (v0 + j >= a.n) continue;
+        const int64_t v = q[j];
         sv[j] = a.seen[v * W + wi];
         for (int l = lo; l < hi; l++) {
           if (!((a.fbits[(int64_t)(l % a.H) * a.fwords + (v >> 5)] >> (v & 31)) & 1u)) continue;
@@ -938,7 +979,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       // sum_t 2^t * popcount(tf & ballot(bit t of deg)).
       int dg[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) dg[j] = v0 + j < a.n ? out_deg(a, (int32_t)(v0 + j)) : 0;
+      for (int j = 0; j < 4; j++) dg[j] = v0 + j < a.n ? out_deg(a, (int32_t)q[j]) : 0;
       // Degree bits of this wave's vertices only (hubs are rare).
       const int dmax = wave_max_i32(max(max(dg[0], dg[1]), max(dg[2], dg[3])));
       const int dbits = dmax > 0 ? 32 - __builtin_clz((unsigned)dmax) : 0;
@@ -1018,10 +1059,12 @@ MsArgs make_args(dawn_graph* g) {
   a.H = kHist;
   a.npad = ws->npad;
   a.m = g->m;
-  a.row_ptr = g->row_ptr;
-  a.col = g->col;
-  a.in_row_ptr = g->in_row_ptr;
-  a.in_col = g->in_col;
+  const MsView* vw = g->view;
+  a.row_ptr = vw ? vw->row_ptr : g->row_ptr;
+  a.col = vw ? vw->col : g->col;
+  a.in_row_ptr = vw ? vw->in_row_ptr : g->in_row_ptr;
+  a.in_col = vw ? vw->in_col : g->in_col;
+  a.perm = vw ? vw->perm : nullptr;
   a.seen = ws->seen.p;
   a.next = ws->next.p;
   a.fbits = ws->fbits.p;
@@ -1031,14 +1074,14 @@ MsArgs make_args(dawn_graph* g) {
   a.q = ws->q.p;
   a.qb = ws->qb.p;
   a.qcap = ws->qcap;
-  a.hc_v = g->hc_v;
-  a.hc_beg = g->hc_beg;
-  a.hc_end = g->hc_end;
-  a.num_hc = g->num_hc;
-  a.oc_v = g->oc_v;
-  a.oc_beg = g->oc_beg;
-  a.oc_end = g->oc_end;
-  a.num_oc = g->num_oc;
+  a.hc_v = vw ? vw->hc_v : g->hc_v;
+  a.hc_beg = vw ? vw->hc_beg : g->hc_beg;
+  a.hc_end = vw ? vw->hc_end : g->hc_end;
+  a.num_hc = vw ? vw->num_hc : g->num_hc;
+  a.oc_v = vw ? vw->oc_v : g->oc_v;
+  a.oc_beg = vw ? vw->oc_beg : g->oc_beg;
+  a.oc_end = vw ? vw->oc_end : g->oc_end;
+  a.num_oc = vw ? vw->num_oc : g->num_oc;
   a.heavy = g->knobs.heavy_degree;
   a.variant = g->knobs.pull_variant;
   a.ring = ws->ring.p;
@@ -1080,8 +1123,9 @@ unsigned ms_final_grid(const dawn_graph* g, int W) {
   return grid;
 }
 
-// Stages one batch's sources (taken mod n) into the pinned batch block.
-void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
+// Stages one batch's sources (taken mod n, then to their position in the
+// hub-first view if there is one) into the pinned batch block.
+void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes, const MsView* view) {
   MsBatch& b = *ws->h_batch;
   std::memset(&b, 0, sizeof b);
   b.nlanes = lanes;
@@ -1093,7 +1137,7 @@ void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes) {
   for (int i = 0; i < lanes; i++) {
     int64_t s = sources[i] % (int64_t)ws->n;
     if (s < 0) s += ws->n;
-    b.src[i] = (int32_t)s;
+    b.src[i] = view ? view->h_perm[s] : (int32_t)s;
   }
 }
 
@@ -1113,6 +1157,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
                 int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
   constexpr int kB = 64 * W;
   DAWN_TRY(ensure_ms(g, W));
+  DAWN_TRY(ms_view_ensure(g));
   MsWorkspace* ws = g->ms;
   hipStream_t s = g->stream;
   const bool dist_dev = dist && is_device_pointer(dist);
@@ -1163,7 +1208,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : ws->distbuf.p;
     hipGraphExec_t exec;
     DAWN_TRY(get_graph(want_edges, &exec));
-    stage_batch(ws, sources + b0, lanes);
+    stage_batch(ws, sources + b0, lanes, g->view);
     ws->h_batch->dist = out_base;
     ws->h_batch->stride = g->n;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
@@ -1224,7 +1269,8 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
         stats->vertices_reached += (int64_t)h_ls[kMaxLanes + i];
         stats->edges_reached += (int64_t)h_ls[2 * kMaxLanes + i];
         const int32_t sv = ws->h_batch->src[i];
-        if (g->h_row_ptr[sv] == g->h_row_ptr[sv + 1]) stats->skipped_sources += 1;
+        const std::vector<int32_t>& rp = g->view ? g->view->h_row_ptr : g->h_row_ptr;
+        if (rp[sv] == rp[sv + 1]) stats->skipped_sources += 1;
       }
       const int levels = ws->h_state->levels;
       for (int st = 1; st <= levels && st < kTraceMax; st++) {
@@ -1247,6 +1293,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
                     int32_t* rows_out, int32_t policy) {
   constexpr int kB = 64 * W;
   DAWN_TRY(ensure_ms(g, W));
+  DAWN_TRY(ms_view_ensure(g));
   MsWorkspace* ws = g->ms;
   const size_t words = (size_t)ws->npad * W;
   DAWN_TRY(ws->seen_bak.alloc(words));
@@ -1263,7 +1310,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
   int32_t rows = 0;
   for (int32_t b0 = 0; b0 < S && rows < max_rows; b0 += kB) {
     const int32_t lanes = std::min<int32_t>(kB, S - b0);
-    stage_batch(ws, sources + b0, lanes);
+    stage_batch(ws, sources + b0, lanes, g->view);
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o);
     ms_seed_kernel<W><<<1, kMaxLanes, 0, s>>>(a);
@@ -1322,6 +1369,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
                      double* kernel_ms, int64_t* launches, int32_t* levels_out) {
   constexpr int kB = 64 * W;
   DAWN_TRY(ensure_ms(g, W));
+  DAWN_TRY(ms_view_ensure(g));
   MsWorkspace* ws = g->ms;
   DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
   hipStream_t s = g->stream;
@@ -1354,7 +1402,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
   int batch = 0;
   for (int32_t b0 = 0; b0 < S; b0 += kB, batch++) {
     const int32_t lanes = std::min<int32_t>(kB, S - b0);
-    stage_batch(ws, sources + b0, lanes);
+    stage_batch(ws, sources + b0, lanes, g->view);
     ws->h_batch->dist = ws->distbuf.p;
     ws->h_batch->stride = g->n;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
