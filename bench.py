@@ -40,6 +40,74 @@ def batch_bytes(n, m, levels, words=1):
     return 4 * (n + 1) + 4 * m + 8 * words * m + 16 * words * n * levels + 256 * words * n
 
 
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "msbfs_rmat24_traffic.json")
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, calibrated on the plain copy), from the committed summary
+    written by --summarize-rocprof; None when there is none for it."""
+    if not os.path.exists(TRAFFIC_PATH):
+        return None
+    t = json.load(open(TRAFFIC_PATH))
+    return t.get("bytes_per_launch", {}).get(kernel)
+
+
+def summarize_rocprof(out_dir, profiles_dir):
+    """Turns the CSV output of the rocprofv3 runs of this benchmark (kernel
+    trace, FETCH_SIZE and WRITE_SIZE of `--quick-profile`, the same two
+    counters of `--probe-only`) into the committed summaries: per-kernel
+    times, per-dispatch traffic of the expand kernel's pull steps (fixed
+    schedule: steps 3-6 of each 24-dispatch batch graph), and the copy
+    calibration factors."""
+    import csv
+    import glob
+
+    def rows(sub, pat):
+        f = glob.glob(os.path.join(out_dir, sub, "**", pat), recursive=True)
+        return list(csv.DictReader(open(f[0]))) if f else []
+
+    def short(name):
+        return name.replace("dawn::(anonymous namespace)::", "").replace("(anonymous namespace)::", "").split("(")[0]
+
+    stats = rows("prof_kt", "*kernel_stats.csv")
+    fetch = rows("prof_fetch", "*counter_collection.csv")
+    write = rows("prof_write", "*counter_collection.csv")
+    cf = rows("prof_cf", "*counter_collection.csv")
+    cw = rows("prof_cw", "*counter_collection.csv")
+    copy_bytes = 1 << 30  # dawn_probe_copy(1 GiB): reads 1 GiB, writes 1 GiB per dispatch
+
+    def per_dispatch(rs, kernel_sub):
+        return [float(r["Counter_Value"]) * 1024.0 for r in rs if kernel_sub in r["Kernel_Name"]]
+
+    fetch_factor = copy_bytes / (sum(per_dispatch(cf, "copy_probe")) / len(per_dispatch(cf, "copy_probe")))
+    write_factor = copy_bytes / (sum(per_dispatch(cw, "copy_probe")) / len(per_dispatch(cw, "copy_probe")))
+    fx, wx = per_dispatch(fetch, "ms_expand_kernel"), per_dispatch(write, "ms_expand_kernel")
+    pulls = [i for i in range(len(fx)) if 2 <= i % 24 <= 5]
+    pull_fetch = sum(fx[i] for i in pulls) / len(pulls) * fetch_factor
+    pull_write = sum(wx[i] for i in pulls) / len(pulls) * write_factor
+    ff, fw = per_dispatch(fetch, "ms_final_kernel"), per_dispatch(write, "ms_final_kernel")
+    final_fetch = sum(ff) / len(ff) * fetch_factor
+    final_write = sum(fw) / len(fw) * write_factor
+    os.makedirs(profiles_dir, exist_ok=True)
+    summary = {
+        "workload": "RMAT scale 24 ef 16 seed 99, 4096 sources, fixed schedule (bench.py --quick-profile)",
+        "fetch_factor": fetch_factor,
+        "write_factor": write_factor,
+        "bytes_per_launch": {"expand_pull": pull_fetch + pull_write, "final": final_fetch + final_write},
+        "expand_pull": {"fetch_bytes": pull_fetch, "write_bytes": pull_write, "dispatches": len(pulls)},
+        "final": {"fetch_bytes": final_fetch, "write_bytes": final_write, "dispatches": len(ff)},
+    }
+    with open(os.path.join(profiles_dir, "msbfs_rmat24_traffic.json"), "w") as f:
+        json.dump(summary, f, indent=1)
+    lines = ["| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    for r in stats:
+        lines.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    return summary, "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,12 +122,19 @@ def main():
     ap.add_argument("--retune", action="store_true",
                     help="sweep the launch knobs on the graph instead of loading the tune file")
     ap.add_argument("--tune-out", default=None, help="with --retune: write the tune cache here")
+    ap.add_argument("--summarize-rocprof", default=None, metavar="DIR",
+                    help="summarise the rocprofv3 CSV output in DIR into profiles/ and exit")
     ap.add_argument("--check", type=int, default=16, help="sources checked bit-exactly vs the oracle")
     ap.add_argument("--quick-profile", action="store_true",
                     help="one learned-schedule pass only (for rocprofv3 runs)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the 1 GiB plain copy kernel (rocprofv3 traffic calibration)")
     args = ap.parse_args()
+    if args.summarize_rocprof:
+        summary, table = summarize_rocprof(args.summarize_rocprof, os.path.join(ROOT, "profiles"))
+        print(table)
+        print(json.dumps(summary, indent=1))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -110,7 +185,9 @@ def main():
     log(f"learned network loaded: {learned_loaded}")
 
     if args.quick_profile:
-        _, sums, st = g.mssp(src, policy="learned", want_dist=False, stats=True)
+        # Fixed schedule: the same modes at the same steps every run, so the
+        # per-dispatch counters of the pull steps can be picked out.
+        _, sums, st = g.mssp(src, policy="fixed", want_dist=False, stats=True)
         print(json.dumps({"quick_profile": True, "device_ms": st["device_ms"], "steps": st["steps"]}))
         return
     copy_gbps = dawn.probe_copy(1 << 30, 5)
@@ -145,7 +222,7 @@ def main():
 
     # Per-step oracle-best schedule on the first 4 batches: every mode timed
     # from identical state (dawn_sched_trace), then each policy's choices.
-    sub = src[:256]
+    sub = src[:64 * W]  # one batch at the timed run's width
     regret = {}
     for pol in ("fixed", "learned"):
         modes, feat, times = g.sched_trace(sub, policy=pol)
@@ -198,7 +275,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": traffic_for(dom),
         "avg_launch_ms": kern[dom]["avg_ms"],
         "bytes_per_launch": dom_bytes,
     }

@@ -12,7 +12,7 @@
    learned choice vs the fixed alpha/beta rule vs the per-step best on the
    recorded steps.
 
-Training graphs (seeds 10-40): RMAT scales 16-22 (ef 16), Erdos-Renyi, and
+Training graphs (seeds 10-40): RMAT scales 16-23 (ef 16), Erdos-Renyi, and
 grids with 1% long-range edges.  The held-out RMAT scale 24 graph of the
 benchmark is not used here.
 
@@ -38,7 +38,7 @@ def training_graphs(quick: bool):
     if quick:
         return [("rmat", 14, 10), ("rmat", 15, 11), ("er", 1 << 15, 12), ("grid", 128, 13)]
     gs = []
-    for i, scale in enumerate(range(16, 23)):
+    for i, scale in enumerate(range(16, 24)):
         gs.append(("rmat", scale, 10 + i))
         gs.append(("rmat", scale, 20 + i))
     for i, n in enumerate((1 << 17, 1 << 19, 1 << 21)):
