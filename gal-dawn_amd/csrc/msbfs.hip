@@ -56,6 +56,7 @@
 namespace dawn {
 
 constexpr int kMaxWords = 8;
+typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
 constexpr int kMaxLanes = 64 * kMaxWords;
 constexpr int kHist = 32;        // level slots
 constexpr int kChunk = 24;       // steps per captured chunk (< kHist)
@@ -946,7 +947,9 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
           }
           int32_t* row = dist + (int64_t)(64 * wi + 32 * h + bb) * stride + v0;
           if (full) {
-            *reinterpret_cast<int4*>(row) = make_int4(d[0], d[1], d[2], d[3]);
+            // Rows are not read back here: streaming stores.
+            const v4i32 val = {d[0], d[1], d[2], d[3]};
+            __builtin_nontemporal_store(val, reinterpret_cast<v4i32*>(row));
           } else {
 #pragma unroll
             for (int j = 0; j < 4; j++)
