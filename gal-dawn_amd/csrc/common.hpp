@@ -128,6 +128,8 @@ struct dawn_graph {
   hipStream_t stream = nullptr;
   std::mutex lock;                // one traversal per graph at a time
   dawn::MsWorkspace* ms = nullptr;
+  dawn::MsWorkspace* ms2 = nullptr;   // second batch in flight (stream2)
+  hipStream_t stream2 = nullptr;
   dawn::SsWorkspace* ss = nullptr;
   dawn::WmsWorkspace* wms = nullptr;
   dawn::MsView* view = nullptr;

@@ -210,7 +210,9 @@ int dawn_graph_destroy(dawn_graph* g) {
   if (!g) return DAWN_OK;
   (void)hipSetDevice(g->device);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->stream2) (void)hipStreamSynchronize(g->stream2);
   ms_workspace_free(g->ms);
+  ms_workspace_free(g->ms2);
   ss_workspace_free(g->ss);
   wms_workspace_free(g->wms);
   ms_view_free(g);
@@ -224,6 +226,7 @@ int dawn_graph_destroy(dawn_graph* g) {
   if (g->col) (void)hipFree(g->col);
   if (g->weight) (void)hipFree(g->weight);
   if (g->stream) (void)hipStreamDestroy(g->stream);
+  if (g->stream2) (void)hipStreamDestroy(g->stream2);
   delete g;
   return DAWN_OK;
 }

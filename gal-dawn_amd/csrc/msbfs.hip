@@ -45,8 +45,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -138,9 +140,10 @@ void ss_invalidate_graphs(SsWorkspace* ws);
 // Captured step graphs bake grid sizes and buffer addresses; the autotuner
 // and workspace re-layouts drop them.
 void invalidate_graphs(dawn_graph* g) {
-  if (g->ms) {
-    for (auto& kv : g->ms->graphs) (void)hipGraphExecDestroy(kv.second);
-    g->ms->graphs.clear();
+  for (MsWorkspace* ws : {g->ms, g->ms2}) {
+    if (!ws) continue;
+    for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+    ws->graphs.clear();
   }
   ss_invalidate_graphs(g->ss);
 }
@@ -1016,9 +1019,9 @@ int words_for(const dawn_graph* g, int32_t S) {
   return w;
 }
 
-int ensure_ms(dawn_graph* g, int W) {
-  if (!g->ms) g->ms = new MsWorkspace();
-  MsWorkspace* ws = g->ms;
+int ensure_ms(dawn_graph* g, MsWorkspace*& slot, int W) {
+  if (!slot) slot = new MsWorkspace();
+  MsWorkspace* ws = slot;
   if (ws->n == g->n && ws->words == W && ws->seen.p) return DAWN_OK;
   // New layout: captured graphs hold the old buffers' addresses.
   for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -1055,8 +1058,7 @@ int ensure_ms(dawn_graph* g, int W) {
   return DAWN_OK;
 }
 
-MsArgs make_args(dawn_graph* g) {
-  MsWorkspace* ws = g->ms;
+MsArgs make_args(dawn_graph* g, MsWorkspace* ws) {
   MsArgs a{};
   a.n = g->n;
   a.H = kHist;
@@ -1155,18 +1157,16 @@ void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t 
 // per batch.
 //   dist: caller buffer (S*n int32, host/device) or null
 //   sums/reached/edges: per source (host) or null
+//   Processes batches first, first + step, ... on workspace ws and stream s.
 template <int W>
-int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-                int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t* sources,
+                   int32_t S, int32_t first, int32_t step, int32_t* dist, int64_t* sums,
+                   int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
   constexpr int kB = 64 * W;
-  DAWN_TRY(ensure_ms(g, W));
-  DAWN_TRY(ms_view_ensure(g));
-  MsWorkspace* ws = g->ms;
-  hipStream_t s = g->stream;
   const bool dist_dev = dist && is_device_pointer(dist);
   if (!dist_dev) DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
 
-  MsArgs a = make_args(g);
+  MsArgs a = make_args(g, ws);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
@@ -1206,7 +1206,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   std::vector<unsigned long long> h_ls(3 * kMaxLanes);
   std::vector<int32_t> h_modes(kTraceMax);
   float total_ms = 0.f;
-  for (int32_t b0 = 0; b0 < S; b0 += kB) {
+  for (int32_t b0 = first * kB; b0 < S; b0 += step * kB) {
     const int32_t lanes = std::min<int32_t>(kB, S - b0);
     int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : ws->distbuf.p;
     hipGraphExec_t exec;
@@ -1287,6 +1287,65 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   return DAWN_OK;
 }
 
+// Runs an unweighted multi-source traversal in batches of 64·W sources.
+// With two or more batches a second workspace and stream (kept with the
+// graph) run every other batch from a second host thread, so that one
+// batch's memory-bound distance pass overlaps the other's steps; if the
+// second workspace cannot be allocated, one stream runs them all.  Then
+// stats->device_ms is the wall time of the whole set of batches.
+template <int W>
+int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
+                int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+  constexpr int kB = 64 * W;
+  DAWN_TRY(ensure_ms(g, g->ms, W));
+  DAWN_TRY(ms_view_ensure(g));
+  bool two = S > kB;
+  if (two) {
+    if (!g->stream2 && hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      g->stream2 = nullptr;
+      two = false;
+    }
+    if (two && ensure_ms(g, g->ms2, W) != DAWN_OK) {
+      ms_workspace_free(g->ms2);  // not enough memory for a second batch in flight
+      g->ms2 = nullptr;
+      two = false;
+    }
+  }
+  if (!two)
+    return run_msbfs_slot<W>(g, g->ms, g->stream, sources, S, 0, 1, dist, sums, reached, edges,
+                             policy, stats);
+  dawn_stats st1{};
+  int rc1 = DAWN_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::thread second([&] {
+    if (hipSetDevice(g->device) != hipSuccess) {
+      rc1 = DAWN_ERR_HIP;
+      return;
+    }
+    rc1 = run_msbfs_slot<W>(g, g->ms2, g->stream2, sources, S, 1, 2, dist, sums, reached, edges,
+                            policy, stats ? &st1 : nullptr);
+  });
+  dawn_stats st0{};
+  const int rc0 = run_msbfs_slot<W>(g, g->ms, g->stream, sources, S, 0, 2, dist, sums, reached,
+                                    edges, policy, stats ? &st0 : nullptr);
+  second.join();
+  const double wall_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (rc0 != DAWN_OK) return rc0;
+  if (rc1 != DAWN_OK) return rc1;
+  if (stats) {
+    stats->device_ms += wall_ms;
+    stats->steps += st0.steps + st1.steps;
+    for (int i = 0; i < DAWN_NUM_MODES; i++) stats->mode_steps[i] += st0.mode_steps[i] + st1.mode_steps[i];
+    stats->edges_reached += st0.edges_reached + st1.edges_reached;
+    stats->vertices_reached += st0.vertices_reached + st1.vertices_reached;
+    stats->batches += st0.batches + st1.batches;
+    stats->skipped_sources += st0.skipped_sources + st1.skipped_sources;
+  }
+  return DAWN_OK;
+}
+
 // Per-step mode timings from identical state (scheduler training data).
 // Each step: decide (features), snapshot, then every mode is run `repeats`
 // times from the snapshot and timed with HIP events; the minimum is kept.
@@ -1295,7 +1354,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
                     float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
                     int32_t* rows_out, int32_t policy) {
   constexpr int kB = 64 * W;
-  DAWN_TRY(ensure_ms(g, W));
+  DAWN_TRY(ensure_ms(g, g->ms, W));
   DAWN_TRY(ms_view_ensure(g));
   MsWorkspace* ws = g->ms;
   const size_t words = (size_t)ws->npad * W;
@@ -1303,7 +1362,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
   DAWN_TRY(ws->done_bak.alloc((size_t)(ws->npad / 32)));
   const size_t done_bytes = sizeof(uint32_t) * (size_t)(ws->npad / 32);
   hipStream_t s = g->stream;
-  MsArgs a = make_args(g);
+  MsArgs a = make_args(g, ws);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
@@ -1371,12 +1430,12 @@ template <int W>
 int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
                      double* kernel_ms, int64_t* launches, int32_t* levels_out) {
   constexpr int kB = 64 * W;
-  DAWN_TRY(ensure_ms(g, W));
+  DAWN_TRY(ensure_ms(g, g->ms, W));
   DAWN_TRY(ms_view_ensure(g));
   MsWorkspace* ws = g->ms;
   DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
   hipStream_t s = g->stream;
-  MsArgs a = make_args(g);
+  MsArgs a = make_args(g, ws);
   SchedParams p = make_params(g, policy);
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
   const unsigned grid = ms_grid(g);
