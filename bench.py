@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--retune", action="store_true",
                     help="sweep the launch knobs on the graph instead of loading the tune file")
     ap.add_argument("--tune-out", default=None, help="with --retune: write the tune cache here")
+    ap.add_argument("--knobs", default="",
+                    help="launch knobs to pin after loading/tuning, e.g. lane_words=16,hub_order=1")
     ap.add_argument("--summarize-rocprof", default=None, metavar="DIR",
                     help="summarise the rocprofv3 CSV output in DIR into profiles/ and exit")
     ap.add_argument("--check", type=int, default=16, help="sources checked bit-exactly vs the oracle")
@@ -170,6 +172,8 @@ def main():
         tune_s = time.time() - t0
         if args.tune_out:
             dawn.tune_save(args.tune_out)
+    if args.knobs:
+        g.set_knobs(**{k: int(v) for k, v in (kv.split("=") for kv in args.knobs.split(","))})
     knobs = g.knobs()
     W = knobs["lane_words"]  # 64*W sources per batch (4096 sources: always the knob)
     log(f"launch knobs: {knobs}{f' tuned in {tune_s:.1f}s' if args.retune else ''}")
@@ -247,8 +251,8 @@ def main():
     log(f"schedule: {sched}")
 
     # Profiling pass (same kernels, one launch at a time with HIP events on
-    # the engine's stream) over 8 batches: per-kernel average durations.
-    prof, levels = g.mssp_profile(src[:512], policy="learned")
+    # the engine's stream) over one batch: per-kernel average durations.
+    prof, levels = g.mssp_profile(src[:64 * W], policy="learned")
     kern = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": v[0] / v[1] if v[1] else 0.0}
             for k, v in prof.items()}
     L = float(levels.mean())

@@ -1,5 +1,5 @@
 // Bit-packed unweighted DAWN over 64·W sources at once (batched BOVM) on one
-// HIP device, W = 1, 2, 4 or 8 lane words per vertex.
+// HIP device, W = 1, 2, 4, 8 or 16 lane words per vertex.
 //
 // Replaces the reference's per-source loops DAWN::MSSP_GPU::run
 // (src/algorithm/gpu/mssp.cu:9-51), DAWN::APSP_GPU::run
@@ -9,13 +9,16 @@
 // 64·i + b.  A vertex's W words are contiguous, so every random access of
 // a step (one frontier gather, one seen probe, one atomic) moves the lanes
 // of all W words in one or two cache lines: the dense steps, bound by
-// random accesses, cost far less per source at W = 8 than at W = 1.
+// random accesses, cost far less per source at W = 8 than at W = 1.  At
+// W = 16 a record is one 128-byte cache line, split over two lanes (eight
+// words each) wherever one lane would hold a whole record.
 //
 // Layout in HBM (per graph, reused across batches; v-major, W words each):
 //   seen   u64[npad][W]        lanes that have reached v
 //   next   u64[npad][W]        scatter target of the step (kept all-zero
 //                              between steps by the finalize pass)
-//   hist   u64[H][npad][W]     level history: slot l%H holds the lanes that
+//   hist   u64[H][npad][W]     level history (H = 32 slots, 16 at W = 16):
+//                              slot l%H holds the lanes that
 //                              reached v exactly at level l (= frontier of l),
 //                              valid only where fbits of that level is set
 //                              (entries of other vertices are never read)
@@ -57,12 +60,22 @@
 
 namespace dawn {
 
-constexpr int kMaxWords = 8;
+constexpr int kMaxWords = 16;
 typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
 constexpr int kMaxLanes = 64 * kMaxWords;
-constexpr int kHist = 32;        // level slots
-constexpr int kChunk = 24;       // steps per captured chunk (< kHist)
-constexpr int kSlices = 5;       // bits of a relative level (2^5 >= kHist)
+constexpr int kSlices = 5;       // bits of a relative level (2^5 >= level slots)
+
+// Level slots of the history ring: 32, or 16 at W = 16 (128-byte records:
+// 16 slots are 34 GB at n = 2^24).
+constexpr int hist_slots(int W) { return W > 8 ? 16 : 32; }
+// Steps per captured chunk.  A chunk writes levels up to `chunk` past the
+// newest level kept from the previous chunk, so chunk + 1 <= slots.
+constexpr int chunk_steps(int W) { return W > 8 ? 14 : 24; }
+static_assert(chunk_steps(16) + 1 <= hist_slots(16) && chunk_steps(8) + 1 <= hist_slots(8), "ring");
+// Words of a record one lane handles in the dense steps: W, or 8 at W = 16
+// (two lanes per vertex, each holding half of the record).
+template <int W>
+constexpr int lane_words_of() { return W > 8 ? 8 : W; }
 
 struct MsBatch {
   int32_t nlanes;
@@ -117,6 +130,7 @@ struct MsOut {
 struct MsWorkspace {
   int32_t n = 0;
   int32_t words = 0;   // W the buffers are laid out for
+  int32_t slots = 0;   // level slots of hist / fbits
   int64_t npad = 0;
   DevBuf<uint64_t> seen, next, hist, seen_bak;
   DevBuf<uint32_t> fbits, done, done_bak;
@@ -197,21 +211,34 @@ __device__ __forceinline__ void or_words(uint64_t* p, const uint64_t (&x)[W]) {
     if (x[i]) atomicOr((unsigned long long*)&p[i], (unsigned long long)x[i]);
 }
 
+// OR over the lanes of the wave that are congruent to this one mod T (a
+// power of two): T = 1 is the whole wave.
+template <int T>
+__device__ __forceinline__ uint64_t wave_or_mod(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o >= T; o >>= 1) x |= shfl_xor_u64(x, o);
+  return x;
+}
+
 // Per-thread counters of a level, reduced over the workgroup and flushed
 // with one set of atomics per workgroup (same-address atomics from every
-// wave serialise in L2).  flush() must be called by every thread.
-template <int W>
+// wave serialise in L2).  alive holds NA of the W lane words: words 0..W-1
+// (NA = W), or words sub*NA .. sub*NA + NA-1 when W/NA threads share a
+// record and this thread holds part `sub`.  flush() must be called by every
+// thread.
+template <int W, int NA>
 struct MsAcc {
   unsigned long long fv = 0, fe = 0, scout = 0, lanes = 0, uv = 0, ue = 0;
-  uint64_t alive[W] = {};
-  __device__ void flush(LevelCounters* c) {
+  uint64_t alive[NA] = {};
+  __device__ void flush(LevelCounters* c, int sub) {
     constexpr int R = 6 + W;
     __shared__ unsigned long long s_v[kBlock / kWave][R];
     const int w = threadIdx.x / kWave;
     unsigned long long r[R] = {wave_sum_u64(fv),    wave_sum_u64(fe), wave_sum_u64(scout),
                                wave_sum_u64(lanes), wave_sum_u64(uv), wave_sum_u64(ue)};
 #pragma unroll
-    for (int i = 0; i < W; i++) r[6 + i] = wave_or_u64(alive[i]);
+    for (int i = 0; i < W; i++)
+      r[6 + i] = wave_or_u64(NA == W || i / NA == sub ? alive[i % NA] : 0ull);
     if (lane_id() == 0)
       for (int i = 0; i < R; i++) s_v[w][i] = r[i];
     __syncthreads();
@@ -333,25 +360,34 @@ __global__ void ms_decide_kernel(MsArgs a, SchedParams p, SchedTrace tr) {
   sched_decide(a.st, a.ring, p, tr);
 }
 
-// Light vertices of a pull step (in-degree < a.heavy), one lane each: U
-// column reads, then U bitmap probes, then the gathers they admit, between
-// early-exit checks (all live missing lanes found).  NT: column reads as
-// streaming loads, so the in-edge lists do not displace the frontier words
-// and bitmap in the caches.  The lane owns next[v].
+// Light vertices of a pull step (in-degree < a.heavy), one lane each (two
+// at W = 16, each with half of the record): U column reads, then U bitmap
+// probes, then the gathers they admit, between early-exit checks (all live
+// missing lanes found).  NT: column reads as streaming loads, so the
+// in-edge lists do not displace the frontier words and bitmap in the
+// caches.  The lane owns its words of next[v].  alive: the live lanes of
+// this lane's words.  The two halves of a W = 16 record are independent:
+// each lane stops once its own half is complete (more gathers could only
+// add lanes outside its miss mask), so a pair needs no shuffles and its
+// loads coalesce into one cache line while both halves are still open.
 template <int W, int U, bool NT>
 __device__ __forceinline__ void pull_light_lanes(const MsArgs& a, const uint64_t* prev,
-                                                 const uint32_t* fb, const uint64_t (&alive)[W]) {
+                                                 const uint32_t* fb,
+                                                 const uint64_t (&alive)[lane_words_of<W>()]) {
+  constexpr int HW = lane_words_of<W>();
+  constexpr int TPV = W / HW;  // lanes per vertex
   const int lane = lane_id();
-  const int64_t wstride = (int64_t)gridDim.x * kBlock;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
-       base += wstride) {
-    const int32_t v = (int32_t)(base + lane);
+  const int h = lane % TPV;
+  const int64_t vstride = (int64_t)gridDim.x * (kBlock / TPV);
+  for (int64_t base = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * (kWave / TPV);
+       base < a.n; base += vstride) {
+    const int32_t v = (int32_t)(base + lane / TPV);
     if (v >= a.n) continue;
-    uint64_t miss[W], sv[W];
-    load_words<W>(a.seen + (int64_t)v * W, sv);
+    uint64_t miss[HW], sv[HW];
+    load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
     uint64_t any = 0;
 #pragma unroll
-    for (int i = 0; i < W; i++) {
+    for (int i = 0; i < HW; i++) {
       miss[i] = ~sv[i] & alive[i];
       any |= miss[i];
     }
@@ -360,18 +396,18 @@ __device__ __forceinline__ void pull_light_lanes(const MsArgs& a, const uint64_t
     const int deg = a.in_row_ptr[v + 1] - beg;
     if (deg >= a.heavy || deg == 0) continue;
     const int32_t* nb = a.in_col + beg;
-    uint64_t acc[W] = {};
+    uint64_t acc[HW] = {};
     auto complete = [&]() {
       bool c = true;
 #pragma unroll
-      for (int i = 0; i < W; i++) c = c && (acc[i] & miss[i]) == miss[i];
+      for (int i = 0; i < HW; i++) c = c && (acc[i] & miss[i]) == miss[i];
       return c;
     };
     auto gather = [&](int32_t u) {
-      uint64_t f[W];
-      load_words<W>(prev + (int64_t)u * W, f);
+      uint64_t f[HW];
+      load_words<HW>(prev + (int64_t)u * W + h * HW, f);
 #pragma unroll
-      for (int i = 0; i < W; i++) acc[i] |= f[i];
+      for (int i = 0; i < HW; i++) acc[i] |= f[i];
     };
     int j = 0;
     for (; j + U <= deg; j += U) {
@@ -391,24 +427,29 @@ __device__ __forceinline__ void pull_light_lanes(const MsArgs& a, const uint64_t
         const int32_t u = NT ? __builtin_nontemporal_load(nb + j) : nb[j];
         if ((fb[u >> 5] >> (u & 31)) & 1u) gather(u);
       }
-    uint64_t* nx = a.next + (int64_t)v * W;
+    uint64_t* nx = a.next + (int64_t)v * W + h * HW;
 #pragma unroll
-    for (int i = 0; i < W; i++)
+    for (int i = 0; i < HW; i++)
       if (acc[i] & miss[i]) nx[i] = acc[i] & miss[i];
   }
 }
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
+  // HW words of a record per lane; at W = 16 lane pairs share a vertex
+  // (TPV = 2), lane l holding words h*HW .. h*HW + HW-1, h = l % TPV.
+  constexpr int HW = lane_words_of<W>();
+  constexpr int TPV = W / HW;
   __shared__ int s_off[kBlock];
   __shared__ int s_beg[kBlock];
-  __shared__ uint64_t s_f[kBlock * W];
+  __shared__ uint64_t s_f[kBlock * HW];
   __shared__ int s_scan[kBlock / kWave];
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
   const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad * W;
   const int lane = lane_id();
+  const int h = lane % TPV;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
 
@@ -419,81 +460,87 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
     for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
       const int64_t i = c0 + threadIdx.x;
       int beg = 0, deg = 0;
-      uint64_t f[W] = {};
+      int32_t u = 0;
       if (i < qn) {
-        const int32_t u = q_in[i];
+        u = q_in[i];
         beg = qb_in[i];
         deg = min(a.row_ptr[u + 1] - beg, kPushChunk);
-        load_words<W>(prev + (int64_t)u * W, f);
       }
       int total;
       const int off = block_exclusive_scan(deg, s_scan, &total);
       s_off[threadIdx.x] = off;
       s_beg[threadIdx.x] = beg;
+      // One pass over the entries' edges per HW-word part of the records.
+      for (int part = 0; part < TPV; part++) {
+        uint64_t f[HW] = {};
+        if (i < qn) load_words<HW>(prev + (int64_t)u * W + part * HW, f);
 #pragma unroll
-      for (int k = 0; k < W; k++) s_f[threadIdx.x * W + k] = f[k];
-      __syncthreads();
-      for (int e = threadIdx.x; e < total; e += kBlock) {
-        const int j = owner_of(s_off, e);
-        const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
-        uint64_t sv[W], nv[W];
-        load_words<W>(a.seen + (int64_t)v * W, sv);
+        for (int k = 0; k < HW; k++) s_f[threadIdx.x * HW + k] = f[k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < total; e += kBlock) {
+          const int j = owner_of(s_off, e);
+          const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
+          uint64_t sv[HW], nv[HW];
+          load_words<HW>(a.seen + (int64_t)v * W + part * HW, sv);
 #pragma unroll
-        for (int k = 0; k < W; k++) nv[k] = s_f[j * W + k] & ~sv[k];
-        or_words<W>(a.next + (int64_t)v * W, nv);
+          for (int k = 0; k < HW; k++) nv[k] = s_f[j * HW + k] & ~sv[k];
+          or_words<HW>(a.next + (int64_t)v * W + part * HW, nv);
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
   } else if (st.mode == DAWN_MODE_BITMAP) {
-    // Dense top-down: light frontier vertices one lane each, heavy ones in
-    // out-edge chunks of knobs.pull_chunk edges, one wave per chunk.
+    // Dense top-down: light frontier vertices one lane (pair) each, heavy
+    // ones in out-edge chunks of knobs.pull_chunk edges, one wave per chunk.
     const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
-    const int64_t wstride = (int64_t)gridDim.x * kBlock;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
-         base += wstride) {
-      const int32_t u = (int32_t)(base + lane);
+    const int64_t vstride = (int64_t)gridDim.x * (kBlock / TPV);
+    for (int64_t base = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * (kWave / TPV);
+         base < a.n; base += vstride) {
+      const int32_t u = (int32_t)(base + lane / TPV);
       if (u >= a.n || !((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
       const int beg = a.row_ptr[u];
       const int deg = a.row_ptr[u + 1] - beg;
       if (deg >= a.heavy) continue;
-      uint64_t f[W];
-      load_words<W>(prev + (int64_t)u * W, f);
+      uint64_t f[HW];
+      load_words<HW>(prev + (int64_t)u * W + h * HW, f);
       for (int j = 0; j < deg; j++) {
         const int32_t v = a.col[beg + j];
-        uint64_t sv[W], nv[W];
-        load_words<W>(a.seen + (int64_t)v * W, sv);
+        uint64_t sv[HW], nv[HW];
+        load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
 #pragma unroll
-        for (int k = 0; k < W; k++) nv[k] = f[k] & ~sv[k];
-        or_words<W>(a.next + (int64_t)v * W, nv);
+        for (int k = 0; k < HW; k++) nv[k] = f[k] & ~sv[k];
+        or_words<HW>(a.next + (int64_t)v * W + h * HW, nv);
       }
     }
     const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t c = wave; c < a.num_oc; c += nwaves) {
+    for (int64_t c = wave; c < a.num_oc; c += nwaves) {  // wave-uniform
       const int32_t u = a.oc_v[c];
       if (!((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
-      uint64_t f[W];
-      load_words<W>(prev + (int64_t)u * W, f);
+      uint64_t f[HW];
+      load_words<HW>(prev + (int64_t)u * W + h * HW, f);
       uint64_t any = 0;
 #pragma unroll
-      for (int k = 0; k < W; k++) any |= f[k];
+      for (int k = 0; k < HW; k++) any |= f[k];
+      if constexpr (TPV > 1) any |= shfl_xor_u64(any, 1);
       if (!any) continue;
       const int e = a.oc_end[c];
-      for (int k = a.oc_beg[c] + lane; k < e; k += kWave) {
+      for (int k = a.oc_beg[c] + lane / TPV; k < e; k += kWave / TPV) {
         const int32_t v = a.col[k];
-        uint64_t sv[W], nv[W];
-        load_words<W>(a.seen + (int64_t)v * W, sv);
+        uint64_t sv[HW], nv[HW];
+        load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
 #pragma unroll
-        for (int t = 0; t < W; t++) nv[t] = f[t] & ~sv[t];
-        or_words<W>(a.next + (int64_t)v * W, nv);
+        for (int t = 0; t < HW; t++) nv[t] = f[t] & ~sv[t];
+        or_words<HW>(a.next + (int64_t)v * W + h * HW, nv);
       }
     }
   } else {  // DAWN_MODE_PULL
     // Only lanes alive in the previous level can arrive now: a vertex looks
     // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
-    uint64_t alive[W];
+    // alive: the live lanes of this lane's HW words.
+    uint64_t alive[HW];
 #pragma unroll
-    for (int k = 0; k < W; k++) alive[k] = a.ring[(lv - 1) % kRing].alive[k];
+    for (int k = 0; k < HW; k++) alive[k] = a.ring[(lv - 1) % kRing].alive[h * HW + k];
     // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
     // frontier words are gathered only for in-neighbours that are in it.
     const uint32_t* fb = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
@@ -503,7 +550,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       // lists are concatenated (degree scan) and read one edge per thread,
       // coalesced; hits are OR-ed into the owner's LDS words.  Gathers stop
       // for an owner once it has found all its missing lanes.
-      // (W = 8 falls back to variant 0: this array would halve occupancy.)
+      // (W >= 8 runs variant 3 instead: this array would halve occupancy.)
       __shared__ unsigned long long s_acc[kBlock * (W <= 4 ? W : 1)];
       for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += wstride) {
         const int32_t v = (int32_t)(b0 + threadIdx.x);
@@ -514,7 +561,7 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
           load_words<W>(a.seen + (int64_t)v * W, sv);
 #pragma unroll
           for (int k = 0; k < W; k++) {
-            miss[k] = ~sv[k] & alive[k];
+            miss[k] = ~sv[k] & alive[k % HW];
             any |= miss[k];
           }
         }
@@ -571,53 +618,62 @@ __global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
       pull_light_lanes<W, (W > 4 ? 4 : 8), true>(a, prev, fb, alive);
     }
     // Heavy vertices: their in-edge lists are cut into chunks of at most
-    // knobs.pull_chunk edges, one wave per chunk, merged with atomicOr.
+    // knobs.pull_chunk edges, one wave per chunk (lane pairs per edge at
+    // W = 16), merged with atomicOr.
     const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t c = wave; c < a.num_hc; c += nwaves) {
+    for (int64_t c = wave; c < a.num_hc; c += nwaves) {  // wave-uniform
       const int32_t v = a.hc_v[c];
-      uint64_t sv[W], nx[W], miss[W];
-      load_words<W>(a.seen + (int64_t)v * W, sv);
-      load_words<W>(a.next + (int64_t)v * W, nx);
+      uint64_t sv[HW], nx[HW], miss[HW];
+      load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
+      load_words<HW>(a.next + (int64_t)v * W + h * HW, nx);
       bool found = true;
       uint64_t any = 0;
 #pragma unroll
-      for (int k = 0; k < W; k++) {
+      for (int k = 0; k < HW; k++) {
         miss[k] = ~sv[k] & alive[k];
         any |= miss[k];
         found = found && (nx[k] & miss[k]) == miss[k];  // other chunks found them all
       }
+      if constexpr (TPV > 1) {
+        any |= shfl_xor_u64(any, 1);
+        found = __shfl_xor((int)found, 1) && found;
+      }
       if (!any || found) continue;
       const int he = a.hc_end[c];
-      uint64_t part[W] = {};
-      for (int k = a.hc_beg[c] + lane; k < he; k += kWave) {
+      uint64_t part[HW] = {};
+      for (int k = a.hc_beg[c] + lane / TPV; k < he; k += kWave / TPV) {
         const int32_t u = a.in_col[k];
         if ((fb[u >> 5] >> (u & 31)) & 1u) {
-          uint64_t f[W];
-          load_words<W>(prev + (int64_t)u * W, f);
+          uint64_t f[HW];
+          load_words<HW>(prev + (int64_t)u * W + h * HW, f);
 #pragma unroll
-          for (int t = 0; t < W; t++) part[t] |= f[t];
+          for (int t = 0; t < HW; t++) part[t] |= f[t];
         }
       }
 #pragma unroll
-      for (int t = 0; t < W; t++) part[t] = wave_or_u64(part[t]) & miss[t];
-      if (lane == 0) or_words<W>(a.next + (int64_t)v * W, part);
+      for (int t = 0; t < HW; t++) part[t] = wave_or_mod<TPV>(part[t]) & miss[t];
+      if (lane < TPV) or_words<HW>(a.next + (int64_t)v * W + h * HW, part);
     }
   }
 }
 
 // Folds next[] into the level slot, seen, the frontier bitmap, the done
-// bitmap and the counters.  Each thread takes four consecutive words with
-// two 16-byte loads per array: 4/W whole vertices (W <= 4), or a quarter
-// of one vertex's record (W = 8: two threads per vertex, combined with a
-// lane shuffle).  npad is a multiple of 64, so a thread's vertices never
-// straddle it.  Vertices already done are skipped after one bitmap read.
+// bitmap and the counters.  Each thread takes WPT consecutive words (4, or
+// 8 at W = 16) with 16-byte loads per array: 4/W whole vertices (W < 4), one
+// vertex (W = 4), or half of one vertex's record (W = 8, 16: two threads
+// per vertex, combined with a lane shuffle).  npad is a multiple of 64, so
+// a thread's vertices never straddle it.  Vertices already done are
+// skipped after one bitmap read.
 template <int W>
 __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
-  constexpr int TPV = W > 4 ? W / 4 : 1;       // threads per vertex
-  constexpr int V = W < 4 ? 4 / W : 1;         // vertices per thread
+  constexpr int WPT = W > 8 ? 8 : 4;           // words per thread
+  constexpr int TPV = W > WPT ? W / WPT : 1;   // threads per vertex
+  constexpr int V = W < WPT ? WPT / W : 1;     // vertices per thread
+  constexpr int WPV = W < WPT ? W : WPT;       // this thread's words of one vertex
   constexpr int LPW = 32 * TPV / V;            // lanes per 32-bit bitmap word
   constexpr unsigned kAll = (1u << V) - 1;
+  static_assert(LPW <= kWave, "a bitmap word is written by one wave");
   const SchedState st = *a.st;
   if (st.done) return;
   const int lv = st.step;
@@ -625,34 +681,42 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   uint32_t* fb = a.fbits + (int64_t)(lv % a.H) * a.fwords;
   LevelCounters* c_out = a.ring + (lv % kRing);
   const int lane = lane_id();
-  const int sub = lane % TPV;  // quarter of the record (W = 8)
-  // Live lanes of the previous level for this thread's words (word i of a
-  // vertex record for W <= 4; words 4*sub+i for W = 8).
-  uint64_t alive_prev[4];
+  const int sub = lane % TPV;  // half of the record (W >= 8)
+  // Live lanes of the previous level for this thread's words of a vertex
+  // (record words sub*WPT + i).
+  uint64_t alive_prev[WPV];
+  const unsigned long long* al = a.ring[(lv - 1) % kRing].alive;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const unsigned long long* al = a.ring[(lv - 1) % kRing].alive;
-    alive_prev[i] = W > 4 ? (sub ? al[4 + i] : al[i]) : (i < W ? al[i] : 0ull);
-  }
-  MsAcc<W> acc;
+  for (int i = 0; i < WPV; i++) alive_prev[i] = al[sub * WPT + i];
+  MsAcc<W, WPV> acc;
   constexpr int64_t kTile = (int64_t)kBlock / TPV * V;
   for (int64_t b0 = (int64_t)blockIdx.x * kTile; b0 < a.npad; b0 += (int64_t)gridDim.x * kTile) {
     const int64_t v0 = b0 + (int64_t)(threadIdx.x / TPV) * V;
     unsigned dnib = kAll;
     if (v0 < a.npad) dnib = (a.done[v0 >> 5] >> (v0 & 31)) & kAll;
     const bool work = v0 < a.npad && dnib != kAll;  // same for the threads of a vertex
-    const int64_t w0 = v0 * W + 4 * sub;
+    const int64_t w0 = v0 * W + WPT * sub;
     ulonglong2* np2 = reinterpret_cast<ulonglong2*>(a.next + w0);
     ulonglong2* sp2 = reinterpret_cast<ulonglong2*>(a.seen + w0);
     ulonglong2* cp2 = reinterpret_cast<ulonglong2*>(cur + w0);
-    uint64_t nx[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, nw[4] = {0, 0, 0, 0};
-    if (work) {
-      const ulonglong2 n01 = np2[0], n23 = np2[1];
-      const ulonglong2 s01 = sp2[0], s23 = sp2[1];
-      nx[0] = n01.x; nx[1] = n01.y; nx[2] = n23.x; nx[3] = n23.y;
-      sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
+    uint64_t sv[WPT], nw[WPT];
+    unsigned nzp = 0;  // 16-byte pairs of next[] that hold bits (to clear)
 #pragma unroll
-      for (int k = 0; k < 4; k++) nw[k] = nx[k] & ~sv[k];
+    for (int k = 0; k < WPT; k++) sv[k] = nw[k] = 0;
+    if (work) {
+      ulonglong2 nn[WPT / 2], ss[WPT / 2];
+#pragma unroll
+      for (int p = 0; p < WPT / 2; p++) nn[p] = np2[p];
+#pragma unroll
+      for (int p = 0; p < WPT / 2; p++) ss[p] = sp2[p];
+#pragma unroll
+      for (int p = 0; p < WPT / 2; p++) {
+        if (nn[p].x | nn[p].y) nzp |= 1u << p;
+        sv[2 * p] = ss[p].x;
+        sv[2 * p + 1] = ss[p].y;
+        nw[2 * p] = nn[p].x & ~ss[p].x;
+        nw[2 * p + 1] = nn[p].y & ~ss[p].y;
+      }
     }
     // Per vertex: arrivals, missing live lanes, popcount (this thread's part).
     uint64_t any[V], unv[V];
@@ -662,19 +726,13 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       any[j] = 0;
       unv[j] = 0;
       pc[j] = 0;
-      constexpr int WPV = W > 4 ? 4 : W;  // this thread's words of one vertex
 #pragma unroll
       for (int i = 0; i < WPV; i++) {
         const uint64_t x = nw[j * WPV + i];
         any[j] |= x;
         pc[j] += (unsigned)__popcll(x);
         unv[j] |= ~(sv[j * WPV + i] | x) & alive_prev[i];
-        if constexpr (W > 4) {
-          acc.alive[i] |= sub ? 0ull : x;
-          acc.alive[4 + i] |= sub ? x : 0ull;
-        } else {
-          acc.alive[i] |= x;
-        }
+        acc.alive[i] |= x;
       }
     }
     if constexpr (TPV > 1) {
@@ -687,20 +745,19 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
     }
     unsigned nib = 0;
     if (work) {
-      if (nx[0] | nx[1]) np2[0] = make_ulonglong2(0ull, 0ull);
-      if (nx[2] | nx[3]) np2[1] = make_ulonglong2(0ull, 0ull);
+#pragma unroll
+      for (int p = 0; p < WPT / 2; p++)
+        if ((nzp >> p) & 1u) np2[p] = make_ulonglong2(0ull, 0ull);
       // Level words only where something arrived (the frontier bitmap marks
       // which entries of the slot are valid), but always a vertex's whole
-      // record.
-      const bool half0 = W >= 4 ? any[0] != 0 : (nw[0] | nw[1]) != 0;
-      const bool half1 = W >= 4 ? any[0] != 0 : (nw[2] | nw[3]) != 0;
-      if (half0) {
-        cp2[0] = make_ulonglong2(nw[0], nw[1]);
-        sp2[0] = make_ulonglong2(sv[0] | nw[0], sv[1] | nw[1]);
-      }
-      if (half1) {
-        cp2[1] = make_ulonglong2(nw[2], nw[3]);
-        sp2[1] = make_ulonglong2(sv[2] | nw[2], sv[3] | nw[3]);
+      // record (W >= 4); for W < 4 a 16-byte pair covers whole vertices.
+#pragma unroll
+      for (int p = 0; p < WPT / 2; p++) {
+        const bool wr = W >= 4 ? any[0] != 0 : (nw[2 * p] | nw[2 * p + 1]) != 0;
+        if (wr) {
+          cp2[p] = make_ulonglong2(nw[2 * p], nw[2 * p + 1]);
+          sp2[p] = make_ulonglong2(sv[2 * p] | nw[2 * p], sv[2 * p + 1] | nw[2 * p + 1]);
+        }
       }
       // Padding vertices (v >= n) have seen = ~0 and next = 0: never new,
       // never unvisited.
@@ -740,7 +797,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
       a.done[v0 >> 5] = y;
     }
   }
-  acc.flush(c_out);
+  acc.flush(c_out, sub);
 }
 
 // Builds the frontier queue of the previous level from its bitmap, only
@@ -779,6 +836,9 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
 // Wave w of a workgroup handles word w % W of a 256-vertex tile (4 vertices
 // per lane), so the 64 rows of one word are written with 16-byte stores,
 // 1 KB per wave per row, and the W waves of a tile share its cache lines.
+// For W > 4 the W/4 workgroups of a tile are placed on one XCD (workgroup
+// ids x, x + 8, ... of the round-robin dispatch), so each record line is
+// fetched into one L2, not into W/4 of them.
 template <int W>
 __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int from_state,
                                                           int lo_arg, int hi_arg, int final_arg) {
@@ -803,16 +863,20 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   }
   __syncthreads();
   // Levels are stored relative to lo in K bit slices: S_k = OR of the
-  // level words whose (l - lo) has bit k set.  hi - lo <= kHist.
+  // level words whose (l - lo) has bit k set.  hi - lo <= H <= 2^kSlices.
   int K = 0;
   while ((1 << K) < hi - lo) K++;
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   // Lane word of this wave, and its tile: W <= 4: the workgroup's four
   // waves cover 4/W tiles; W = 8: two workgroups share a tile.
-  const int wi = W > 4 ? (int)(blockIdx.x % BPT) * 4 + wv : wv % W;
+  // W > 4: the grid is a multiple of kXcds * BPT; workgroup x runs on XCD
+  // x % kXcds, and r = x / kXcds counts that XCD's workgroups, BPT per tile.
+  const unsigned xr = blockIdx.x / kXcds;
+  const int wi = W > 4 ? (int)(xr % BPT) * 4 + wv : wv % W;
   const int tile = W > 4 ? 0 : wv / W;
-  const int64_t tile0 = W > 4 ? (int64_t)(blockIdx.x / BPT) : (int64_t)blockIdx.x * TPB + tile;
+  const int64_t tile0 = W > 4 ? (int64_t)(xr / BPT) * kXcds + blockIdx.x % kXcds
+                              : (int64_t)blockIdx.x * TPB + tile;
   const int rows = min(64, nl - 64 * wi);  // lanes of this word in the batch
   // With one flush of every level (lo == 0, final) every (lane, vertex) is
   // written: level or INF.
@@ -1030,14 +1094,20 @@ int ensure_ms(dawn_graph* g, MsWorkspace*& slot, int W) {
   ws->distbuf.release();
   for (DevBuf<uint64_t>* b : {&ws->seen, &ws->next, &ws->hist}) b->release();
   ws->n = g->n;
-  ws->words = W;
+  ws->words = 0;  // set once every buffer is in place
+  ws->slots = hist_slots(W);
   ws->npad = ((int64_t)g->n + 63) / 64 * 64;
   const size_t words = (size_t)ws->npad * W;
-  DAWN_TRY(ws->seen.alloc(words));
-  DAWN_TRY(ws->next.alloc(words));
-  DAWN_TRY(ws->fbits.alloc((size_t)kHist * (ws->npad / 32)));
+  int rc = ws->seen.alloc(words);
+  if (!rc) rc = ws->next.alloc(words);
+  if (!rc) rc = ws->hist.alloc((size_t)ws->slots * words);
+  if (rc) {  // out of memory: drop the partial layout (the next call starts over)
+    for (DevBuf<uint64_t>* b : {&ws->seen, &ws->next, &ws->hist}) b->release();
+    (void)hipGetLastError();
+    return rc;
+  }
+  DAWN_TRY(ws->fbits.alloc((size_t)ws->slots * (ws->npad / 32)));
   DAWN_TRY(ws->done.alloc((size_t)(ws->npad / 32)));
-  DAWN_TRY(ws->hist.alloc((size_t)kHist * words));
   // Queue entries per level: one per kPushChunk out-edges of each frontier
   // vertex, at most n + m / kPushChunk.
   ws->qcap = (int64_t)g->n + g->m / kPushChunk + 64;
@@ -1055,13 +1125,14 @@ int ensure_ms(dawn_graph* g, MsWorkspace*& slot, int W) {
   if (!ws->h_batch) DAWN_HIP(hipHostMalloc((void**)&ws->h_batch, sizeof(MsBatch)));
   for (hipEvent_t* e : {&ws->ev0, &ws->ev1, &ws->e_a, &ws->e_b})
     if (!*e) DAWN_HIP(hipEventCreate(e));
+  ws->words = W;
   return DAWN_OK;
 }
 
 MsArgs make_args(dawn_graph* g, MsWorkspace* ws) {
   MsArgs a{};
   a.n = g->n;
-  a.H = kHist;
+  a.H = ws->slots;
   a.npad = ws->npad;
   a.m = g->m;
   const MsView* vw = g->view;
@@ -1115,16 +1186,20 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
 unsigned ms_grid(const dawn_graph* g) {
   return grid_for(std::max<int64_t>(g->n, 1), kBlock, (unsigned)g->knobs.dense_blocks);
 }
-// Finalize pass: 1024/W vertices per workgroup and iteration; at most 2048
-// workgroups (8 per CU), since each flushes its counters with atomics.
+// Finalize pass: 1024/W vertices per workgroup and iteration (128 at
+// W = 16); at most 2048 workgroups (8 per CU), since each flushes its
+// counters with atomics.
 unsigned ms_fin_grid(const dawn_graph* g, int W) {
-  return grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, 2048u);
+  return grid_for(std::max<int64_t>(g->n, 1), W > 8 ? kBlock / 2 : kBlock * 4 / W, 2048u);
 }
-// Distance pass: 1024/W vertices per workgroup (W = 8: two workgroups per
-// 256-vertex tile, so the grid is even).
+// Distance pass: 1024/W vertices per workgroup (W > 4: W/4 workgroups per
+// 256-vertex tile, on one XCD, so the grid is a multiple of kXcds * W/4).
 unsigned ms_final_grid(const dawn_graph* g, int W) {
   unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, (unsigned)g->knobs.dense_blocks);
-  if (W > 4) grid = std::max(2u, grid & ~1u);
+  if (W > 4) {
+    const unsigned unit = (unsigned)(kXcds * W / 4);
+    grid = std::max(unit, (grid + unit - 1) / unit * unit);
+  }
   return grid;
 }
 
@@ -1177,6 +1252,7 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
   // Graph of one batch: fill, seed, kChunk steps, final pass (no-op unless
   // the traversal finished within the chunk).  Sources and the output
   // pointer live in the device batch block, so one graph serves all batches.
+  constexpr int kChunk = chunk_steps(W);
   auto get_graph = [&](int want_edges, hipGraphExec_t* exec) -> int {
     const auto key = std::make_tuple(W, (int)policy, want_edges, (const void*)p.mlp);
     auto it = ws->graphs.find(key);
@@ -1299,6 +1375,9 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   constexpr int kB = 64 * W;
   DAWN_TRY(ensure_ms(g, g->ms, W));
   DAWN_TRY(ms_view_ensure(g));
+  // Row buffers of batches whose rows do not go to a device buffer.
+  const bool dist_dev = dist && is_device_pointer(dist);
+  if (!dist_dev) DAWN_TRY(g->ms->distbuf.alloc((size_t)kB * g->n));
   bool two = S > kB;
   if (two) {
     if (!g->stream2 && hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking) != hipSuccess) {
@@ -1306,9 +1385,11 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       g->stream2 = nullptr;
       two = false;
     }
-    if (two && ensure_ms(g, g->ms2, W) != DAWN_OK) {
+    if (two && (ensure_ms(g, g->ms2, W) != DAWN_OK ||
+                (!dist_dev && g->ms2->distbuf.alloc((size_t)kB * g->n) != DAWN_OK))) {
       ms_workspace_free(g->ms2);  // not enough memory for a second batch in flight
       g->ms2 = nullptr;
+      (void)hipGetLastError();
       two = false;
     }
   }
@@ -1376,7 +1457,7 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     ms_fill_kernel<W><<<grid_for(ws->npad * W), kBlock, 0, s>>>(a, o);
     ms_seed_kernel<W><<<1, kMaxLanes, 0, s>>>(a);
-    for (int step = 1; rows < max_rows && step < kHist; step++) {
+    for (int step = 1; rows < max_rows && step < a.H; step++) {
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
       DAWN_HIP(hipStreamSynchronize(s));
@@ -1487,7 +1568,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
         ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
       }));
       DAWN_TRY(timed(6, [&] { ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a); }));
-      if (st.step - lo >= kHist - 2) {  // flush before the history ring wraps
+      if (st.step - lo >= a.H - 2) {  // flush before the history ring wraps
         DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
         lo = st.step;
       }
@@ -1508,7 +1589,8 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, i
     case 1: return run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats);
     case 2: return run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats);
     case 4: return run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats);
-    default: return run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats);
+    case 8: return run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats);
+    default: return run_msbfs_w<16>(g, sources, S, dist, sums, reached, edges, policy, stats);
   }
 }
 
@@ -1528,9 +1610,12 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
     case 4:
       return run_ms_record_w<4>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
                                 rows_out, policy);
-    default:
+    case 8:
       return run_ms_record_w<8>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
                                 rows_out, policy);
+    default:
+      return run_ms_record_w<16>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
+                                 rows_out, policy);
   }
 }
 
@@ -1543,7 +1628,8 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
     case 1: return run_ms_profile_w<1>(g, sources, S, policy, kernel_ms, launches, levels_out);
     case 2: return run_ms_profile_w<2>(g, sources, S, policy, kernel_ms, launches, levels_out);
     case 4: return run_ms_profile_w<4>(g, sources, S, policy, kernel_ms, launches, levels_out);
-    default: return run_ms_profile_w<8>(g, sources, S, policy, kernel_ms, launches, levels_out);
+    case 8: return run_ms_profile_w<8>(g, sources, S, policy, kernel_ms, launches, levels_out);
+    default: return run_ms_profile_w<16>(g, sources, S, policy, kernel_ms, launches, levels_out);
   }
 }
 

@@ -19,8 +19,8 @@ struct LevelCounters {
   unsigned long long flanes;  // sum of popcount of frontier words (batched)
   unsigned long long uv;      // vertices still missing a live lane after this level
   unsigned long long ue;      // sum of in-degree over those (pull's edges to check)
-  unsigned long long alive[8];  // batched: OR of the level's frontier words, one
-                                // per 64 lanes (lanes that can still reach new vertices)
+  unsigned long long alive[16];  // batched: OR of the level's frontier words, one
+                                 // per 64 lanes (lanes that can still reach new vertices)
   unsigned int qlen;          // queue entries written for this level
   unsigned int pad;
 };

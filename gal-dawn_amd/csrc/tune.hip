@@ -69,7 +69,8 @@ bool knobs_valid(const TuneKnobs& k) {
   return k.dense_blocks >= 1 && k.dense_blocks <= 65536 && k.heavy_degree >= 1 &&
          k.pull_chunk >= 1 && k.ss_chunk >= 0 && k.ss_chunk <= 1024 && k.pull_variant >= 0 &&
          k.pull_variant <= 3 &&
-         (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4 || k.lane_words == 8) &&
+         (k.lane_words == 1 || k.lane_words == 2 || k.lane_words == 4 || k.lane_words == 8 ||
+          k.lane_words == 16) &&
          (k.hub_order == 0 || k.hub_order == 1);
 }
 
@@ -108,7 +109,7 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
   if (!g || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   // Sample sources: the highest-degree vertex and spread vertices with edges.
   std::vector<int32_t> src;
-  const int want = g->weighted ? 2 : 512;
+  const int want = g->weighted ? 2 : 1024;
   for (int32_t v = 0; v < g->n && (int)src.size() < want; v += std::max(1, g->n / (4 * want)))
     if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
   if (src.empty()) return DAWN_ERR_INVALID_ARG;
@@ -119,7 +120,14 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
   auto consider = [&](TuneKnobs k) -> int {
     DAWN_TRY(apply(g, k));
     double ms = 0;
-    DAWN_TRY(sample_ms(g, src, &ms));
+    const int rc = sample_ms(g, src, &ms);
+    if (rc == DAWN_ERR_OUT_OF_MEMORY) {  // e.g. 1024-lane buffers of a huge graph: not a candidate
+      if (verbose)
+        std::fprintf(stderr, "tune %s words=%d: out of device memory, skipped\n", class_key(g).c_str(),
+                     k.lane_words);
+      return DAWN_OK;
+    }
+    DAWN_TRY(rc);
     if (verbose)
       std::fprintf(stderr,
                    "tune %s blocks=%d heavy=%d chunk=%d ss_chunk=%d variant=%d words=%d hubs=%d: "
@@ -139,13 +147,13 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       DAWN_TRY(consider(k));
     }
   } else {
-    // The same 512 sample sources at every width (8, 4, 2 or 1 batches).
+    // The same 1024 sample sources at every width (1 to 16 batches).
     for (int ho : {0, 1}) {
       TuneKnobs k = best;
       k.hub_order = ho;
       DAWN_TRY(consider(k));
     }
-    for (int lw : {1, 2, 4, 8}) {
+    for (int lw : {1, 2, 4, 8, 16}) {
       TuneKnobs k = best;
       k.lane_words = lw;
       DAWN_TRY(consider(k));

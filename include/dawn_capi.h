@@ -314,7 +314,7 @@ int dawn_tune_save(const char* path);
  * dense batched kernels, degree from which edge lists are cut into wave
  * chunks, chunk length, single-source steps per captured graph (0 = auto),
  * light-pull variant (0..3), 64-lane words per vertex of the batched engine
- * (1, 2, 4 or 8: 64, 128, 256 or 512 sources per batch), hub-first vertex
+ * (1, 2, 4, 8 or 16: 64 to 1024 sources per batch), hub-first vertex
  * order of the batched engine (0 or 1; graphs of 65,536 vertices or more).
  * Knobs change time only, never results.  dawn_graph_set_knobs rejects
  * values out of range. */

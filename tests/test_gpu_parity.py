@@ -207,10 +207,11 @@ def test_tuned_knobs_keep_results_exact(orc, tmp_path):
     assert len(open(p).read().splitlines()) >= 2
 
 
-@pytest.mark.parametrize("words", [1, 2, 4, 8])
+@pytest.mark.parametrize("words", [1, 2, 4, 8, 16])
 def test_lane_words_exact(orc, words):
-    """64, 128, 256 and 512 sources per batch, with a partial last batch and
-    duplicate sources: rows and sums equal the oracle's."""
+    """64, 128, 256, 512 and 1024 sources per batch, with a partial last
+    batch (at 1024: six empty lane words) and duplicate sources: rows and
+    sums equal the oracle's."""
     log(f"case: RMAT scale 13 ef 8 seed 6, 700 sources, lane_words={words}")
     h = dawn.HostGraph.rmat(13, 8, 6)
     g = dawn.Graph(h)
@@ -288,6 +289,29 @@ def test_hub_order_view_exact(orc, directed):
         dist, sums = g.mssp(srcs, policy="fixed")
         assert np.array_equal(dist, odist), hub
         assert np.array_equal(sums, osums), hub
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_wide_batches_hub_view_exact(orc, directed):
+    """1024 sources per batch (records split over lane pairs), two batches in
+    flight on two streams, hub-first view, every step mode: rows and sums
+    equal the oracle's."""
+    log(f"case: RMAT scale 16 ef 8 seed 14, directed={directed}, 1100 sources, lane_words=16")
+    h = dawn.HostGraph.rmat(16, 8, 14)
+    if directed:
+        rows = np.repeat(np.arange(h.n, dtype=np.int32), np.diff(h.row_ptr))
+        keep = rows > h.col
+        h = dawn.HostGraph.from_csr(*_directed_csr(h.n, rows[keep], h.col[keep]), directed=True)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=16, hub_order=1)
+    srcs = dawn.gen_sources(h.n, 1100, 14)
+    srcs[900] = srcs[5]
+    odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    for p in ("fixed", "pull", "push", "bitmap"):
+        dist, sums, st = g.mssp(srcs, policy=p, stats=True)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+        assert st["batches"] == 2 and st["skipped_sources"] == int(skipped.sum())
 
 
 def _directed_csr(n, u, v):
