@@ -147,15 +147,16 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       DAWN_TRY(consider(k));
     }
   } else {
-    // The same 1024 sample sources at every width (1 to 16 batches).
-    for (int ho : {0, 1}) {
-      TuneKnobs k = best;
-      k.hub_order = ho;
-      DAWN_TRY(consider(k));
-    }
+    // The same 1024 sample sources at every width (1 to 16 batches); the
+    // width first, since the best vertex order and launch shapes depend on it.
     for (int lw : {1, 2, 4, 8, 16}) {
       TuneKnobs k = best;
       k.lane_words = lw;
+      DAWN_TRY(consider(k));
+    }
+    for (int ho : {0, 1}) {
+      TuneKnobs k = best;
+      k.hub_order = ho;
       DAWN_TRY(consider(k));
     }
     for (int pv : {0, 1, 2, 3}) {
