@@ -57,9 +57,11 @@ def summarize_rocprof(out_dir, profiles_dir):
     """Turns the CSV output of the rocprofv3 runs of this benchmark (kernel
     trace, FETCH_SIZE and WRITE_SIZE of `--quick-profile`, the same two
     counters of `--probe-only`) into the committed summaries: per-kernel
-    times, per-dispatch traffic of the expand kernel's pull steps (fixed
-    schedule: steps 3-6 of each 24-dispatch batch graph), and the copy
-    calibration factors."""
+    times, per-dispatch traffic of the pull and distance kernels, and the
+    copy calibration factors.  Each step of a captured batch graph launches
+    the push, top-down and pull kernels and all but the chosen one return at
+    once, so per-kernel averages are also given over the dispatches that did
+    work (longer than 20 us)."""
     import csv
     import glob
 
@@ -71,6 +73,7 @@ def summarize_rocprof(out_dir, profiles_dir):
         return name.replace("dawn::(anonymous namespace)::", "").replace("(anonymous namespace)::", "").split("(")[0]
 
     stats = rows("prof_kt", "*kernel_stats.csv")
+    trace = rows("prof_kt", "*kernel_trace.csv")
     fetch = rows("prof_fetch", "*counter_collection.csv")
     write = rows("prof_write", "*counter_collection.csv")
     cf = rows("prof_cf", "*counter_collection.csv")
@@ -80,31 +83,48 @@ def summarize_rocprof(out_dir, profiles_dir):
     def per_dispatch(rs, kernel_sub):
         return [float(r["Counter_Value"]) * 1024.0 for r in rs if kernel_sub in r["Kernel_Name"]]
 
+    def durations_us(kernel_sub):
+        return [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                for r in trace if kernel_sub in r["Kernel_Name"]]
+
     fetch_factor = copy_bytes / (sum(per_dispatch(cf, "copy_probe")) / len(per_dispatch(cf, "copy_probe")))
     write_factor = copy_bytes / (sum(per_dispatch(cw, "copy_probe")) / len(per_dispatch(cw, "copy_probe")))
-    fx, wx = per_dispatch(fetch, "ms_expand_kernel"), per_dispatch(write, "ms_expand_kernel")
-    pulls = [i for i in range(len(fx)) if 2 <= i % 24 <= 5]
-    pull_fetch = sum(fx[i] for i in pulls) / len(pulls) * fetch_factor
-    pull_write = sum(wx[i] for i in pulls) / len(pulls) * write_factor
-    ff, fw = per_dispatch(fetch, "ms_final_kernel"), per_dispatch(write, "ms_final_kernel")
-    final_fetch = sum(ff) / len(ff) * fetch_factor
-    final_write = sum(fw) / len(fw) * write_factor
+
+    def traffic(kernel_sub):
+        # Dispatches that did work: the counter runs replay the same fixed
+        # schedule, so they are the ones with more than 1 MB of traffic.
+        fx, wx = per_dispatch(fetch, kernel_sub), per_dispatch(write, kernel_sub)
+        act = [i for i in range(min(len(fx), len(wx))) if fx[i] + wx[i] > 1e6]
+        if not act:
+            return None
+        f = sum(fx[i] for i in act) / len(act) * fetch_factor
+        w = sum(wx[i] for i in act) / len(act) * write_factor
+        d = [x for x in durations_us(kernel_sub) if x > 20.0]
+        return {"fetch_bytes": f, "write_bytes": w, "dispatches": len(act),
+                "avg_active_us": sum(d) / len(d) if d else None, "active_dispatches_traced": len(d)}
+
+    pull = traffic("ms_pull_kernel")
+    final = traffic("ms_final_kernel")
     os.makedirs(profiles_dir, exist_ok=True)
     summary = {
         "workload": "RMAT scale 24 ef 16 seed 99, 4096 sources, fixed schedule (bench.py --quick-profile)",
         "fetch_factor": fetch_factor,
         "write_factor": write_factor,
-        "bytes_per_launch": {"expand_pull": pull_fetch + pull_write, "final": final_fetch + final_write},
-        "expand_pull": {"fetch_bytes": pull_fetch, "write_bytes": pull_write, "dispatches": len(pulls)},
-        "final": {"fetch_bytes": final_fetch, "write_bytes": final_write, "dispatches": len(ff)},
+        "bytes_per_launch": {"expand_pull": pull["fetch_bytes"] + pull["write_bytes"],
+                             "final": final["fetch_bytes"] + final["write_bytes"]},
+        "expand_pull": pull,
+        "final": final,
     }
     with open(os.path.join(profiles_dir, "msbfs_rmat24_traffic.json"), "w") as f:
         json.dump(summary, f, indent=1)
-    lines = ["| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    lines = ["| kernel | calls | total ms | avg us | min us | max us | % | working calls | avg us of working calls |",
+             "|---|---|---|---|---|---|---|---|---|"]
     for r in stats:
+        d = [x for x in durations_us(r["Name"]) if x > 20.0]
+        act = f"{len(d)} | {sum(d) / len(d):.1f}" if d else "0 | -"
         lines.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
                      f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
-                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} | {act} |")
     return summary, "\n".join(lines)
 
 

@@ -262,25 +262,71 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int o) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Value of lane (this lane ^ J), J a power of two below 64: DPP quad
+// permutes for J = 1, 2, ds_swizzle (bit mode, within 32 lanes) for
+// J = 4..16, ds_bpermute across the halves for J = 32.  All lanes active.
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane_u32(uint32_t x) {
+  static_assert(J >= 1 && J <= 32 && (J & (J - 1)) == 0, "power of two");
+  if constexpr (J == 1) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  } else if constexpr (J == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  } else if constexpr (J < 32) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (J << 10));
+  } else {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ 32u) << 2), (int)x);
+  }
+}
+
+template <int J>
+__device__ __forceinline__ uint64_t xor_lane_u64(uint64_t x) {
+  const uint32_t lo = xor_lane_u32<J>((uint32_t)x);
+  const uint32_t hi = xor_lane_u32<J>((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// OR (or sum) over the lanes l ^ m, m < N (N a power of two, at most 64):
+// the lanes of an aligned group of N.  All lanes active.
+template <int N>
+__device__ __forceinline__ uint32_t group_or_u32(uint32_t x) {
+  if constexpr (N >= 2) return group_or_u32<N / 2>(x | xor_lane_u32<N / 2>(x));
+  else return x;
+}
+template <int N>
+__device__ __forceinline__ uint64_t group_or_u64(uint64_t x) {
+  if constexpr (N >= 2) return group_or_u64<N / 2>(x | xor_lane_u64<N / 2>(x));
+  else return x;
+}
+template <int N>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t x) {
+  if constexpr (N >= 2) return group_sum_u32<N / 2>(x + xor_lane_u32<N / 2>(x));
+  else return x;
+}
+
+// One round of the 64x64 bit transpose: swaps bit J of the row index (the
+// lane) with bit J of the column index.  M: the low J bits of every 2J-bit
+// block.  Branch-free: the lane's own half is kept, the other half comes
+// from lane ^ J shifted into place.
+template <int J, uint64_t M>
+__device__ __forceinline__ uint64_t transpose_round(uint64_t x, int lane) {
+  const uint64_t o = xor_lane_u64<J>(x);
+  const bool upper = (lane & J) != 0;
+  const uint64_t y = upper ? (o >> J) : (o << J);
+  const uint64_t keep = upper ? ~M : M;
+  return (x & keep) | (y & ~keep);
+}
+
 // Transpose the 64x64 bit matrix held one row per lane: afterwards lane j
-// holds bit i = bit j of lane i's input.  Six butterfly rounds; round j
-// swaps bit j of the row index with bit j of the column index.
+// holds bit i = bit j of lane i's input.  Six butterfly rounds.
 __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x) {
   const int l = lane_id();
-#pragma unroll
-  for (int j = 32, r = 0; j >= 1; j >>= 1, r++) {
-    const uint64_t M = (j == 32)  ? 0x00000000FFFFFFFFull
-                       : (j == 16) ? 0x0000FFFF0000FFFFull
-                       : (j == 8)  ? 0x00FF00FF00FF00FFull
-                       : (j == 4)  ? 0x0F0F0F0F0F0F0F0Full
-                       : (j == 2)  ? 0x3333333333333333ull
-                                   : 0x5555555555555555ull;
-    uint64_t o = shfl_xor_u64(x, j);
-    if ((l & j) == 0)
-      x = (x & M) | ((o & M) << j);
-    else
-      x = ((o & ~M) >> j) | (x & ~M);
-  }
+  x = transpose_round<32, 0x00000000FFFFFFFFull>(x, l);
+  x = transpose_round<16, 0x0000FFFF0000FFFFull>(x, l);
+  x = transpose_round<8, 0x00FF00FF00FF00FFull>(x, l);
+  x = transpose_round<4, 0x0F0F0F0F0F0F0F0Full>(x, l);
+  x = transpose_round<2, 0x3333333333333333ull>(x, l);
+  x = transpose_round<1, 0x5555555555555555ull>(x, l);
   return x;
 }
 

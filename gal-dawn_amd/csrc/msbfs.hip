@@ -434,229 +434,286 @@ __device__ __forceinline__ void pull_light_lanes(const MsArgs& a, const uint64_t
   }
 }
 
+// Expansion of one step, one kernel per mode (the device scheduler picks
+// the mode; the other two return at once).  Separate kernels keep each
+// mode's registers and LDS to itself.  HW words of a record per lane; at
+// W = 16 lane pairs share a vertex (TPV = 2), lane l holding words
+// h*HW .. h*HW + HW-1, h = l % TPV.
+#define MS_STEP_PROLOGUE(MODE)                                              \
+  constexpr int HW = lane_words_of<W>();                                    \
+  constexpr int TPV = W / HW;                                               \
+  const SchedState st = *a.st;                                              \
+  if (st.done || st.mode != (MODE)) return;                                 \
+  const int lv = st.step;                                                   \
+  const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad * W;   \
+  const int lane = lane_id();                                               \
+  const int h = lane % TPV;                                                 \
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ring[(lv + 1) % kRing] = LevelCounters{}; \
+  (void)lane;                                                               \
+  (void)h
+
+// PUSH: queue entries, workgroup-balanced expansion (LDS degree scan).
 template <int W>
-__global__ void __launch_bounds__(kBlock) ms_expand_kernel(MsArgs a) {
-  // HW words of a record per lane; at W = 16 lane pairs share a vertex
-  // (TPV = 2), lane l holding words h*HW .. h*HW + HW-1, h = l % TPV.
-  constexpr int HW = lane_words_of<W>();
-  constexpr int TPV = W / HW;
+__global__ void __launch_bounds__(kBlock) ms_push_kernel(MsArgs a) {
+  MS_STEP_PROLOGUE(DAWN_MODE_PUSH);
   __shared__ int s_off[kBlock];
   __shared__ int s_beg[kBlock];
   __shared__ uint64_t s_f[kBlock * HW];
+  __shared__ unsigned s_nz[kBlock];
   __shared__ int s_scan[kBlock / kWave];
-  const SchedState st = *a.st;
-  if (st.done) return;
-  const int lv = st.step;
-  const uint64_t* prev = a.hist + (int64_t)((lv - 1) % a.H) * a.npad * W;
-  const int lane = lane_id();
-  const int h = lane % TPV;
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
+  const int32_t* q_in = a.q;  // built by ms_compact_kernel for this step
+  const int32_t* qb_in = a.qb;
+  const unsigned qn = a.ring[(lv - 1) % kRing].qlen;
+  for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    int beg = 0, deg = 0;
+    int32_t u = 0;
+    if (i < qn) {
+      u = q_in[i];
+      beg = qb_in[i];
+      deg = min(a.row_ptr[u + 1] - beg, kPushChunk);
+    }
+    int total;
+    const int off = block_exclusive_scan(deg, s_scan, &total);
+    s_off[threadIdx.x] = off;
+    s_beg[threadIdx.x] = beg;
+    // One pass over the entries' edges per HW-word part of the records.
+    // Frontier records of sparse levels hold few nonzero words: an edge
+    // touches seen/next only in the words its owner has (8-byte loads
+    // when at most half of them are set, whole-part loads otherwise), and
+    // owners with an empty part skip their edges.
+    for (int part = 0; part < TPV; part++) {
+      uint64_t f[HW] = {};
+      if (i < qn) load_words<HW>(prev + (int64_t)u * W + part * HW, f);
+      unsigned nz = 0;
+#pragma unroll
+      for (int k = 0; k < HW; k++) {
+        s_f[threadIdx.x * HW + k] = f[k];
+        nz |= (f[k] != 0ull ? 1u : 0u) << k;
+      }
+      s_nz[threadIdx.x] = nz;
+      __syncthreads();
+      for (int e = threadIdx.x; e < total; e += kBlock) {
+        const int j = owner_of(s_off, e);
+        unsigned msk = s_nz[j];
+        if (!msk) continue;
+        const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
+        const int64_t vw = (int64_t)v * W + part * HW;
+        if (2 * __popc(msk) > HW) {
+          uint64_t sv[HW], nv[HW];
+          load_words<HW>(a.seen + vw, sv);
+#pragma unroll
+          for (int k = 0; k < HW; k++) nv[k] = s_f[j * HW + k] & ~sv[k];
+          or_words<HW>(a.next + vw, nv);
+        } else {
+          for (; msk; msk &= msk - 1) {
+            const int k = __ffs(msk) - 1;
+            const uint64_t nv = s_f[j * HW + k] & ~a.seen[vw + k];
+            if (nv) atomicOr((unsigned long long*)&a.next[vw + k], (unsigned long long)nv);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
 
-  if (st.mode == DAWN_MODE_PUSH) {
-    const int32_t* q_in = a.q;  // built by ms_compact_kernel for this step
-    const int32_t* qb_in = a.qb;
-    const unsigned qn = a.ring[(lv - 1) % kRing].qlen;
-    for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
-      const int64_t i = c0 + threadIdx.x;
+// BITMAP: dense top-down scan of the frontier bitmap.
+template <int W>
+__global__ void __launch_bounds__(kBlock) ms_topdown_kernel(MsArgs a) {
+  MS_STEP_PROLOGUE(DAWN_MODE_BITMAP);
+  // Dense top-down: light frontier vertices one lane (pair) each, heavy
+  // ones in out-edge chunks of knobs.pull_chunk edges, one wave per chunk.
+  const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
+  const int64_t vstride = (int64_t)gridDim.x * (kBlock / TPV);
+  for (int64_t base = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * (kWave / TPV);
+       base < a.n; base += vstride) {
+    const int32_t u = (int32_t)(base + lane / TPV);
+    if (u >= a.n || !((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
+    const int beg = a.row_ptr[u];
+    const int deg = a.row_ptr[u + 1] - beg;
+    if (deg >= a.heavy) continue;
+    uint64_t f[HW];
+    load_words<HW>(prev + (int64_t)u * W + h * HW, f);
+    int nzc = 0;
+#pragma unroll
+    for (int k = 0; k < HW; k++) nzc += f[k] != 0ull;
+    if (!nzc) continue;
+    const bool dense = 2 * nzc > HW;  // else: 8-byte accesses of the nonzero words only
+    for (int j = 0; j < deg; j++) {
+      const int64_t vw = (int64_t)a.col[beg + j] * W + h * HW;
+      if (dense) {
+        uint64_t sv[HW], nv[HW];
+        load_words<HW>(a.seen + vw, sv);
+#pragma unroll
+        for (int k = 0; k < HW; k++) nv[k] = f[k] & ~sv[k];
+        or_words<HW>(a.next + vw, nv);
+      } else {
+#pragma unroll
+        for (int k = 0; k < HW; k++) {
+          if (!f[k]) continue;
+          const uint64_t nv = f[k] & ~a.seen[vw + k];
+          if (nv) atomicOr((unsigned long long*)&a.next[vw + k], (unsigned long long)nv);
+        }
+      }
+    }
+  }
+  const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t c = wave; c < a.num_oc; c += nwaves) {  // wave-uniform
+    const int32_t u = a.oc_v[c];
+    if (!((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
+    uint64_t f[HW];
+    load_words<HW>(prev + (int64_t)u * W + h * HW, f);
+    uint64_t any = 0;
+#pragma unroll
+    for (int k = 0; k < HW; k++) any |= f[k];
+    if constexpr (TPV > 1) any |= shfl_xor_u64(any, 1);
+    if (!any) continue;
+    const int e = a.oc_end[c];
+    for (int k = a.oc_beg[c] + lane / TPV; k < e; k += kWave / TPV) {
+      const int32_t v = a.col[k];
+      uint64_t sv[HW], nv[HW];
+      load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
+#pragma unroll
+      for (int t = 0; t < HW; t++) nv[t] = f[t] & ~sv[t];
+      or_words<HW>(a.next + (int64_t)v * W + h * HW, nv);
+    }
+  }
+}
+
+// PULL: bottom-up over the in-edges (CSC).  PV: the light-vertex variant
+// (knobs.pull_variant), a template argument so that each variant gets the
+// registers it needs and no more.
+template <int W, int PV>
+__global__ void __launch_bounds__(kBlock) ms_pull_kernel(MsArgs a) {
+  MS_STEP_PROLOGUE(DAWN_MODE_PULL);
+  // Only lanes alive in the previous level can arrive now: a vertex looks
+  // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
+  // alive: the live lanes of this lane's HW words.
+  uint64_t alive[HW];
+#pragma unroll
+  for (int k = 0; k < HW; k++) alive[k] = a.ring[(lv - 1) % kRing].alive[h * HW + k];
+  // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
+  // frontier words are gathered only for in-neighbours that are in it.
+  const uint32_t* fb = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
+  if constexpr (W <= 4 && PV == 1) {
+    // Light vertices, edge-balanced: the workgroup's 256 vertices' in-edge
+    // lists are concatenated (degree scan) and read one edge per thread,
+    // coalesced; hits are OR-ed into the owner's LDS words.  Gathers stop
+    // for an owner once it has found all its missing lanes.
+    // (W >= 8 runs variant 3 instead: these arrays would halve occupancy.)
+    __shared__ int s_off[kBlock];
+    __shared__ int s_beg[kBlock];
+    __shared__ uint64_t s_f[kBlock * W];
+    __shared__ int s_scan[kBlock / kWave];
+    __shared__ unsigned long long s_acc[kBlock * W];
+    const int64_t wstride = (int64_t)gridDim.x * kBlock;
+    for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += wstride) {
+      const int32_t v = (int32_t)(b0 + threadIdx.x);
+      uint64_t miss[W] = {};
+      uint64_t any = 0;
+      if (v < a.n) {
+        uint64_t sv[W];
+        load_words<W>(a.seen + (int64_t)v * W, sv);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+          miss[k] = ~sv[k] & alive[k % HW];
+          any |= miss[k];
+        }
+      }
       int beg = 0, deg = 0;
-      int32_t u = 0;
-      if (i < qn) {
-        u = q_in[i];
-        beg = qb_in[i];
-        deg = min(a.row_ptr[u + 1] - beg, kPushChunk);
+      if (any) {
+        beg = a.in_row_ptr[v];
+        const int d = a.in_row_ptr[v + 1] - beg;
+        deg = d < a.heavy ? d : 0;
       }
       int total;
       const int off = block_exclusive_scan(deg, s_scan, &total);
       s_off[threadIdx.x] = off;
       s_beg[threadIdx.x] = beg;
-      // One pass over the entries' edges per HW-word part of the records.
-      for (int part = 0; part < TPV; part++) {
-        uint64_t f[HW] = {};
-        if (i < qn) load_words<HW>(prev + (int64_t)u * W + part * HW, f);
 #pragma unroll
-        for (int k = 0; k < HW; k++) s_f[threadIdx.x * HW + k] = f[k];
-        __syncthreads();
-        for (int e = threadIdx.x; e < total; e += kBlock) {
-          const int j = owner_of(s_off, e);
-          const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
-          uint64_t sv[HW], nv[HW];
-          load_words<HW>(a.seen + (int64_t)v * W + part * HW, sv);
-#pragma unroll
-          for (int k = 0; k < HW; k++) nv[k] = s_f[j * HW + k] & ~sv[k];
-          or_words<HW>(a.next + (int64_t)v * W + part * HW, nv);
-        }
-        __syncthreads();
+      for (int k = 0; k < W; k++) {
+        s_f[threadIdx.x * W + k] = miss[k];
+        s_acc[threadIdx.x * W + k] = 0ull;
       }
-    }
-  } else if (st.mode == DAWN_MODE_BITMAP) {
-    // Dense top-down: light frontier vertices one lane (pair) each, heavy
-    // ones in out-edge chunks of knobs.pull_chunk edges, one wave per chunk.
-    const uint32_t* fbp = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
-    const int64_t vstride = (int64_t)gridDim.x * (kBlock / TPV);
-    for (int64_t base = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * (kWave / TPV);
-         base < a.n; base += vstride) {
-      const int32_t u = (int32_t)(base + lane / TPV);
-      if (u >= a.n || !((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
-      const int beg = a.row_ptr[u];
-      const int deg = a.row_ptr[u + 1] - beg;
-      if (deg >= a.heavy) continue;
-      uint64_t f[HW];
-      load_words<HW>(prev + (int64_t)u * W + h * HW, f);
-      for (int j = 0; j < deg; j++) {
-        const int32_t v = a.col[beg + j];
-        uint64_t sv[HW], nv[HW];
-        load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
-#pragma unroll
-        for (int k = 0; k < HW; k++) nv[k] = f[k] & ~sv[k];
-        or_words<HW>(a.next + (int64_t)v * W + h * HW, nv);
-      }
-    }
-    const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t c = wave; c < a.num_oc; c += nwaves) {  // wave-uniform
-      const int32_t u = a.oc_v[c];
-      if (!((fbp[u >> 5] >> (u & 31)) & 1u)) continue;
-      uint64_t f[HW];
-      load_words<HW>(prev + (int64_t)u * W + h * HW, f);
-      uint64_t any = 0;
-#pragma unroll
-      for (int k = 0; k < HW; k++) any |= f[k];
-      if constexpr (TPV > 1) any |= shfl_xor_u64(any, 1);
-      if (!any) continue;
-      const int e = a.oc_end[c];
-      for (int k = a.oc_beg[c] + lane / TPV; k < e; k += kWave / TPV) {
-        const int32_t v = a.col[k];
-        uint64_t sv[HW], nv[HW];
-        load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
-#pragma unroll
-        for (int t = 0; t < HW; t++) nv[t] = f[t] & ~sv[t];
-        or_words<HW>(a.next + (int64_t)v * W + h * HW, nv);
-      }
-    }
-  } else {  // DAWN_MODE_PULL
-    // Only lanes alive in the previous level can arrive now: a vertex looks
-    // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
-    // alive: the live lanes of this lane's HW words.
-    uint64_t alive[HW];
-#pragma unroll
-    for (int k = 0; k < HW; k++) alive[k] = a.ring[(lv - 1) % kRing].alive[h * HW + k];
-    // Frontier bitmap of the previous level (2 MB at n = 2^24, L2-resident):
-    // frontier words are gathered only for in-neighbours that are in it.
-    const uint32_t* fb = a.fbits + (int64_t)((lv - 1) % a.H) * a.fwords;
-    const int64_t wstride = (int64_t)gridDim.x * kBlock;
-    if (W <= 4 && a.variant == 1) {
-      // Light vertices, edge-balanced: the workgroup's 256 vertices' in-edge
-      // lists are concatenated (degree scan) and read one edge per thread,
-      // coalesced; hits are OR-ed into the owner's LDS words.  Gathers stop
-      // for an owner once it has found all its missing lanes.
-      // (W >= 8 runs variant 3 instead: this array would halve occupancy.)
-      __shared__ unsigned long long s_acc[kBlock * (W <= 4 ? W : 1)];
-      for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += wstride) {
-        const int32_t v = (int32_t)(b0 + threadIdx.x);
-        uint64_t miss[W] = {};
-        uint64_t any = 0;
-        if (v < a.n) {
-          uint64_t sv[W];
-          load_words<W>(a.seen + (int64_t)v * W, sv);
+      __syncthreads();
+      for (int e = threadIdx.x; e < total; e += kBlock) {
+        const int j = owner_of(s_off, e);
+        const int32_t u = a.in_col[s_beg[j] + (e - s_off[j])];
+        if ((fb[u >> 5] >> (u & 31)) & 1u) {
+          bool done = true;
 #pragma unroll
           for (int k = 0; k < W; k++) {
-            miss[k] = ~sv[k] & alive[k % HW];
-            any |= miss[k];
+            const uint64_t mj = s_f[j * W + k];
+            done = done && (__atomic_load_n(&s_acc[j * W + k], __ATOMIC_RELAXED) & mj) == mj;
           }
-        }
-        int beg = 0, deg = 0;
-        if (any) {
-          beg = a.in_row_ptr[v];
-          const int d = a.in_row_ptr[v + 1] - beg;
-          deg = d < a.heavy ? d : 0;
-        }
-        int total;
-        const int off = block_exclusive_scan(deg, s_scan, &total);
-        s_off[threadIdx.x] = off;
-        s_beg[threadIdx.x] = beg;
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-          s_f[threadIdx.x * W + k] = miss[k];
-          s_acc[threadIdx.x * W + k] = 0ull;
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < total; e += kBlock) {
-          const int j = owner_of(s_off, e);
-          const int32_t u = a.in_col[s_beg[j] + (e - s_off[j])];
-          if ((fb[u >> 5] >> (u & 31)) & 1u) {
-            bool done = true;
+          if (!done) {
+            uint64_t f[W];
+            load_words<W>(prev + (int64_t)u * W, f);
 #pragma unroll
             for (int k = 0; k < W; k++) {
-              const uint64_t mj = s_f[j * W + k];
-              done = done && (__atomic_load_n(&s_acc[j * W + k], __ATOMIC_RELAXED) & mj) == mj;
-            }
-            if (!done) {
-              uint64_t f[W];
-              load_words<W>(prev + (int64_t)u * W, f);
-#pragma unroll
-              for (int k = 0; k < W; k++) {
-                const uint64_t hit = f[k] & s_f[j * W + k];
-                if (hit) atomicOr(&s_acc[j * W + k], (unsigned long long)hit);
-              }
+              const uint64_t hit = f[k] & s_f[j * W + k];
+              if (hit) atomicOr(&s_acc[j * W + k], (unsigned long long)hit);
             }
           }
         }
-        __syncthreads();
-        if (deg) {
-#pragma unroll
-          for (int k = 0; k < W; k++)
-            if (s_acc[threadIdx.x * W + k]) a.next[(int64_t)v * W + k] = s_acc[threadIdx.x * W + k];
-        }
       }
-    } else if (a.variant == 0) {
-      // Gathers in flight per lane: fewer for wide records (registers).
-      pull_light_lanes<W, (W > 4 ? 2 : 4), false>(a, prev, fb, alive);
-    } else if (a.variant == 2) {
-      pull_light_lanes<W, (W > 4 ? 4 : 8), false>(a, prev, fb, alive);
-    } else {
-      pull_light_lanes<W, (W > 4 ? 4 : 8), true>(a, prev, fb, alive);
+      __syncthreads();
+      if (deg) {
+#pragma unroll
+        for (int k = 0; k < W; k++)
+          if (s_acc[threadIdx.x * W + k]) a.next[(int64_t)v * W + k] = s_acc[threadIdx.x * W + k];
+      }
     }
-    // Heavy vertices: their in-edge lists are cut into chunks of at most
-    // knobs.pull_chunk edges, one wave per chunk (lane pairs per edge at
-    // W = 16), merged with atomicOr.
-    const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t c = wave; c < a.num_hc; c += nwaves) {  // wave-uniform
-      const int32_t v = a.hc_v[c];
-      uint64_t sv[HW], nx[HW], miss[HW];
-      load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
-      load_words<HW>(a.next + (int64_t)v * W + h * HW, nx);
-      bool found = true;
-      uint64_t any = 0;
+  } else if constexpr (PV == 0) {
+    // Gathers in flight per lane: fewer for wide records (registers).
+    pull_light_lanes<W, (W > 4 ? 2 : 4), false>(a, prev, fb, alive);
+  } else if constexpr (PV == 2) {
+    pull_light_lanes<W, (W > 4 ? 4 : 8), false>(a, prev, fb, alive);
+  } else {
+    pull_light_lanes<W, (W > 4 ? 4 : 8), true>(a, prev, fb, alive);
+  }
+  // Heavy vertices: their in-edge lists are cut into chunks of at most
+  // knobs.pull_chunk edges, one wave per chunk (lane pairs per edge at
+  // W = 16), merged with atomicOr.
+  const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t c = wave; c < a.num_hc; c += nwaves) {  // wave-uniform
+    const int32_t v = a.hc_v[c];
+    uint64_t sv[HW], nx[HW], miss[HW];
+    load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
+    load_words<HW>(a.next + (int64_t)v * W + h * HW, nx);
+    bool found = true;
+    uint64_t any = 0;
 #pragma unroll
-      for (int k = 0; k < HW; k++) {
-        miss[k] = ~sv[k] & alive[k];
-        any |= miss[k];
-        found = found && (nx[k] & miss[k]) == miss[k];  // other chunks found them all
-      }
-      if constexpr (TPV > 1) {
-        any |= shfl_xor_u64(any, 1);
-        found = __shfl_xor((int)found, 1) && found;
-      }
-      if (!any || found) continue;
-      const int he = a.hc_end[c];
-      uint64_t part[HW] = {};
-      for (int k = a.hc_beg[c] + lane / TPV; k < he; k += kWave / TPV) {
-        const int32_t u = a.in_col[k];
-        if ((fb[u >> 5] >> (u & 31)) & 1u) {
-          uint64_t f[HW];
-          load_words<HW>(prev + (int64_t)u * W + h * HW, f);
-#pragma unroll
-          for (int t = 0; t < HW; t++) part[t] |= f[t];
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < HW; t++) part[t] = wave_or_mod<TPV>(part[t]) & miss[t];
-      if (lane < TPV) or_words<HW>(a.next + (int64_t)v * W + h * HW, part);
+    for (int k = 0; k < HW; k++) {
+      miss[k] = ~sv[k] & alive[k];
+      any |= miss[k];
+      found = found && (nx[k] & miss[k]) == miss[k];  // other chunks found them all
     }
+    if constexpr (TPV > 1) {
+      any |= shfl_xor_u64(any, 1);
+      found = __shfl_xor((int)found, 1) && found;
+    }
+    if (!any || found) continue;
+    const int he = a.hc_end[c];
+    uint64_t part[HW] = {};
+    for (int k = a.hc_beg[c] + lane / TPV; k < he; k += kWave / TPV) {
+      const int32_t u = a.in_col[k];
+      if ((fb[u >> 5] >> (u & 31)) & 1u) {
+        uint64_t f[HW];
+        load_words<HW>(prev + (int64_t)u * W + h * HW, f);
+#pragma unroll
+        for (int t = 0; t < HW; t++) part[t] |= f[t];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < HW; t++) part[t] = wave_or_mod<TPV>(part[t]) & miss[t];
+    if (lane < TPV) or_words<HW>(a.next + (int64_t)v * W + h * HW, part);
   }
 }
+#undef MS_STEP_PROLOGUE
 
 // Folds next[] into the level slot, seen, the frontier bitmap, the done
 // bitmap and the counters.  Each thread takes WPT consecutive words (4, or
@@ -735,13 +792,10 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
         acc.alive[i] |= x;
       }
     }
-    if constexpr (TPV > 1) {
-#pragma unroll
-      for (int o = 1; o < TPV; o <<= 1) {
-        any[0] |= shfl_xor_u64(any[0], o);
-        unv[0] |= shfl_xor_u64(unv[0], o);
-        pc[0] += __shfl_xor(pc[0], o);
-      }
+    if constexpr (TPV > 1) {  // every lane of the wave is here
+      any[0] = group_or_u64<TPV>(any[0]);
+      unv[0] = group_or_u64<TPV>(unv[0]);
+      pc[0] = group_sum_u32<TPV>(pc[0]);
     }
     unsigned nib = 0;
     if (work) {
@@ -785,13 +839,8 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
     if (sub != 0) dnib = 0;  // the first thread of a vertex reports its bits
     // Frontier and done bitmaps: LPW lanes (32 vertices) per word.
     const int pos = ((lane / TPV) * V) & 31;
-    unsigned x = nib << pos;
-    unsigned y = dnib << pos;
-#pragma unroll
-    for (int o = 1; o < LPW; o <<= 1) {
-      x |= __shfl_xor(x, o);
-      y |= __shfl_xor(y, o);
-    }
+    const unsigned x = group_or_u32<LPW>(nib << pos);
+    const unsigned y = group_or_u32<LPW>(dnib << pos);
     if ((lane % LPW) == 0 && v0 < a.npad) {
       fb[v0 >> 5] = x;
       a.done[v0 >> 5] = y;
@@ -901,8 +950,11 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       const int64_t v = v0 + j;
       q[j] = ident || v >= a.n ? v : (int64_t)a.perm[v];
     }
+    // seen decides which unreached rows get INF in partial writes; whole
+    // 16-byte row stores (vec_ok) write every entry and do not need it.
     if (v0 + 3 < a.n && ident) {
-      if constexpr (W == 1) {
+      if (vec_ok) {
+      } else if constexpr (W == 1) {
         const ulonglong2* sp = reinterpret_cast<const ulonglong2*>(a.seen + v0);
         const ulonglong2 s01 = sp[0], s23 = sp[1];
         sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
@@ -937,8 +989,10 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         }
       }
     } else if (v0 + 3 < a.n) {
+      if (!vec_ok) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) sv[j] = a.seen[q[j] * W + wi];
+        for (int j = 0; j < 4; j++) sv[j] = a.seen[q[j] * W + wi];
+      }
       // Outside the hub range the four positions are consecutive: their
       // bitmap bits come from one (or two adjacent) words.
       const bool run = q[1] == q[0] + 1 && q[2] == q[0] + 2 && q[3] == q[0] + 3;
@@ -1222,9 +1276,21 @@ void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes, const M
 }
 
 template <int W>
+void launch_pull(const MsArgs& a, unsigned grid, hipStream_t s) {
+  switch (a.variant) {
+    case 0: ms_pull_kernel<W, 0><<<grid, kBlock, 0, s>>>(a); break;
+    case 1: ms_pull_kernel<W, 1><<<grid, kBlock, 0, s>>>(a); break;
+    case 2: ms_pull_kernel<W, 2><<<grid, kBlock, 0, s>>>(a); break;
+    default: ms_pull_kernel<W, 3><<<grid, kBlock, 0, s>>>(a); break;
+  }
+}
+
+template <int W>
 void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t s) {
   ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
-  ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
+  ms_push_kernel<W><<<grid, kBlock, 0, s>>>(a);
+  ms_topdown_kernel<W><<<grid, kBlock, 0, s>>>(a);
+  launch_pull<W>(a, grid, s);
   ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
 }
 
@@ -1564,8 +1630,14 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
       }
       const int slot = st.mode == DAWN_MODE_PUSH ? 3 : st.mode == DAWN_MODE_PULL ? 4 : 5;
       DAWN_TRY(timed(slot, [&] {
-        ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
-        ms_expand_kernel<W><<<grid, kBlock, 0, s>>>(a);
+        if (st.mode == DAWN_MODE_PUSH) {
+          ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
+          ms_push_kernel<W><<<grid, kBlock, 0, s>>>(a);
+        } else if (st.mode == DAWN_MODE_PULL) {
+          launch_pull<W>(a, grid, s);
+        } else {
+          ms_topdown_kernel<W><<<grid, kBlock, 0, s>>>(a);
+        }
       }));
       DAWN_TRY(timed(6, [&] { ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a); }));
       if (st.step - lo >= a.H - 2) {  // flush before the history ring wraps
