@@ -205,6 +205,17 @@ __device__ __forceinline__ void load_words(const uint64_t* p, uint64_t (&x)[W]) 
 }
 
 template <int W>
+__device__ __forceinline__ void store_words(uint64_t* p, const uint64_t (&x)[W]) {
+  if constexpr (W == 1) {
+    p[0] = x[0];
+  } else {
+    ulonglong2* q = reinterpret_cast<ulonglong2*>(p);
+#pragma unroll
+    for (int i = 0; i < W / 2; i++) q[i] = make_ulonglong2(x[2 * i], x[2 * i + 1]);
+  }
+}
+
+template <int W>
 __device__ __forceinline__ void or_words(uint64_t* p, const uint64_t (&x)[W]) {
 #pragma unroll
   for (int i = 0; i < W; i++)
@@ -213,11 +224,21 @@ __device__ __forceinline__ void or_words(uint64_t* p, const uint64_t (&x)[W]) {
 
 // OR over the lanes of the wave that are congruent to this one mod T (a
 // power of two): T = 1 is the whole wave.
-template <int T>
+template <int T, int J = 32>
 __device__ __forceinline__ uint64_t wave_or_mod(uint64_t x) {
-#pragma unroll
-  for (int o = 32; o >= T; o >>= 1) x |= shfl_xor_u64(x, o);
-  return x;
+  if constexpr (J >= T) return wave_or_mod<T, J / 2>(x | xor_lane_u64<J>(x));
+  else return x;
+}
+
+// Bits 0, 2, 4, ... of a 64-bit mask, packed into 32 bits.
+__device__ __forceinline__ uint32_t even_bits(uint64_t x) {
+  x &= 0x5555555555555555ull;
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  return (uint32_t)x;
 }
 
 // Per-thread counters of a level, reduced over the workgroup and flushed
@@ -360,78 +381,194 @@ __global__ void ms_decide_kernel(MsArgs a, SchedParams p, SchedTrace tr) {
   sched_decide(a.st, a.ring, p, tr);
 }
 
+// Pull steps fold their light vertices in the pull kernel, except the
+// edge-balanced variant (1, W <= 4), which leaves every vertex to finalize.
+template <int W>
+__device__ __forceinline__ bool pull_folds_light(const MsArgs& a) {
+  return !(W <= 4 && a.variant == 1);
+}
+
 // Light vertices of a pull step (in-degree < a.heavy), one lane each (two
-// at W = 16, each with half of the record): U column reads, then U bitmap
-// probes, then the gathers they admit, between early-exit checks (all live
-// missing lanes found).  NT: column reads as streaming loads, so the
-// in-edge lists do not displace the frontier words and bitmap in the
-// caches.  The lane owns its words of next[v].  alive: the live lanes of
-// this lane's words.  The two halves of a W = 16 record are independent:
-// each lane stops once its own half is complete (more gathers could only
-// add lanes outside its miss mask), so a pair needs no shuffles and its
-// loads coalesce into one cache line while both halves are still open.
+// at W = 16, each with half of the record), with the step's fold done in
+// place, so that next[] is not written, read back and cleared for them:
+// the lane writes its vertex's level record and seen words, and the wave
+// writes the frontier and done bitmap words of its 64 (32) vertices and
+// accumulates the level counters.  Heavy vertices (chunked below, merged in
+// next[]) are folded by the finalize pass; their bitmap bits are left
+// clear here (a done heavy vertex keeps its done bit).
+//
+// Per lane: U column reads, then U bitmap probes, then the gathers they
+// admit, between early-exit checks (all live missing lanes found).  NT:
+// column reads as streaming loads.  The halves of a W = 16 record are
+// independent: each lane stops once its own half is complete (more
+// gathers could only add lanes outside its miss mask).  Vertices marked
+// done are skipped before their seen words are read.  seen keeps only the
+// live lanes: alive & ~(still missing) -- bits of lanes that are no longer
+// alive are never read again (miss, the push masks and the fold only look
+// at live lanes; distances come from the level records).
 template <int W, int U, bool NT>
-__device__ __forceinline__ void pull_light_lanes(const MsArgs& a, const uint64_t* prev,
+__device__ __forceinline__ void pull_light_fused(const MsArgs& a, int lv, const uint64_t* prev,
                                                  const uint32_t* fb,
                                                  const uint64_t (&alive)[lane_words_of<W>()]) {
   constexpr int HW = lane_words_of<W>();
-  constexpr int TPV = W / HW;  // lanes per vertex
+  constexpr int TPV = W / HW;       // lanes per vertex
+  constexpr int VPW = kWave / TPV;  // vertices per wave and pass: two bitmap words, or one
   const int lane = lane_id();
   const int h = lane % TPV;
+  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
+  uint32_t* fbo = a.fbits + (int64_t)(lv % a.H) * a.fwords;
+  // Level counters: per-lane 32-bit sums (each bounded by m + n < 2^32),
+  // vertex counts from ballots, live lanes OR-ed into LDS (checked first,
+  // so the atomics stop once the words fill up).
+  __shared__ unsigned long long s_alive[W];
+  __shared__ unsigned long long s_cnt[kBlock / kWave][6];
+  if (threadIdx.x < W) s_alive[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned c_fe = 0, c_scout = 0, c_ue = 0, c_lanes = 0;
+  unsigned long long c_fv = 0, c_uv = 0;  // wave-uniform
   const int64_t vstride = (int64_t)gridDim.x * (kBlock / TPV);
-  for (int64_t base = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * (kWave / TPV);
-       base < a.n; base += vstride) {
+  for (int64_t base = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * VPW;
+       base < a.npad; base += vstride) {  // wave-uniform: the whole wave runs every pass
     const int32_t v = (int32_t)(base + lane / TPV);
-    if (v >= a.n) continue;
-    uint64_t miss[HW], sv[HW];
-    load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
-    uint64_t any = 0;
-#pragma unroll
-    for (int i = 0; i < HW; i++) {
-      miss[i] = ~sv[i] & alive[i];
-      any |= miss[i];
+    const bool done_before = v >= a.n || ((a.done[v >> 5] >> (v & 31)) & 1u);
+    int beg = 0, deg = 0;
+    if (!done_before) {
+      beg = a.in_row_ptr[v];
+      deg = a.in_row_ptr[v + 1] - beg;
     }
-    if (!any) continue;
-    const int beg = a.in_row_ptr[v];
-    const int deg = a.in_row_ptr[v + 1] - beg;
-    if (deg >= a.heavy || deg == 0) continue;
-    const int32_t* nb = a.in_col + beg;
-    uint64_t acc[HW] = {};
-    auto complete = [&]() {
-      bool c = true;
+    const bool light = !done_before && deg < a.heavy;
+    uint64_t nw[HW];
 #pragma unroll
-      for (int i = 0; i < HW; i++) c = c && (acc[i] & miss[i]) == miss[i];
-      return c;
-    };
-    auto gather = [&](int32_t u) {
-      uint64_t f[HW];
-      load_words<HW>(prev + (int64_t)u * W + h * HW, f);
+    for (int i = 0; i < HW; i++) nw[i] = 0;
+    unsigned still = 0;  // this half still misses a live lane after the step
+    if (light) {
+      uint64_t miss[HW];
+      {
+        uint64_t sv[HW];
+        load_words<HW>(a.seen + (int64_t)v * W + h * HW, sv);
 #pragma unroll
-      for (int i = 0; i < HW; i++) acc[i] |= f[i];
-    };
-    int j = 0;
-    for (; j + U <= deg; j += U) {
-      int32_t u[U];
-      uint32_t w[U];
-#pragma unroll
-      for (int t = 0; t < U; t++) u[t] = NT ? __builtin_nontemporal_load(nb + j + t) : nb[j + t];
-#pragma unroll
-      for (int t = 0; t < U; t++) w[t] = fb[u[t] >> 5];
-#pragma unroll
-      for (int t = 0; t < U; t++)
-        if ((w[t] >> (u[t] & 31)) & 1u) gather(u[t]);
-      if (complete()) break;
-    }
-    if (!complete())
-      for (; j < deg; j++) {
-        const int32_t u = NT ? __builtin_nontemporal_load(nb + j) : nb[j];
-        if ((fb[u >> 5] >> (u & 31)) & 1u) gather(u);
+        for (int i = 0; i < HW; i++) miss[i] = ~sv[i] & alive[i];
       }
-    uint64_t* nx = a.next + (int64_t)v * W + h * HW;
+      uint64_t any = 0;
 #pragma unroll
-    for (int i = 0; i < HW; i++)
-      if (acc[i] & miss[i]) nx[i] = acc[i] & miss[i];
+      for (int i = 0; i < HW; i++) any |= miss[i];
+      if (any && deg > 0) {
+        const int32_t* nb = a.in_col + beg;
+        uint64_t acc_w[HW] = {};
+        auto complete = [&]() {
+          bool c = true;
+#pragma unroll
+          for (int i = 0; i < HW; i++) c = c && (acc_w[i] & miss[i]) == miss[i];
+          return c;
+        };
+        auto gather = [&](int32_t u) {
+          uint64_t f[HW];
+          load_words<HW>(prev + (int64_t)u * W + h * HW, f);
+#pragma unroll
+          for (int i = 0; i < HW; i++) acc_w[i] |= f[i];
+        };
+        int j = 0;
+        for (; j + U <= deg; j += U) {
+          int32_t u[U];
+          uint32_t w[U];
+#pragma unroll
+          for (int t = 0; t < U; t++) u[t] = NT ? __builtin_nontemporal_load(nb + j + t) : nb[j + t];
+#pragma unroll
+          for (int t = 0; t < U; t++) w[t] = fb[u[t] >> 5];
+#pragma unroll
+          for (int t = 0; t < U; t++)
+            if ((w[t] >> (u[t] & 31)) & 1u) gather(u[t]);
+          if (complete()) break;
+        }
+        if (!complete())
+          for (; j < deg; j++) {
+            const int32_t u = NT ? __builtin_nontemporal_load(nb + j) : nb[j];
+            if ((fb[u >> 5] >> (u & 31)) & 1u) gather(u);
+          }
+#pragma unroll
+        for (int i = 0; i < HW; i++) nw[i] = acc_w[i] & miss[i];
+      }
+      uint64_t rest = 0, got = 0;
+#pragma unroll
+      for (int i = 0; i < HW; i++) {
+        rest |= miss[i] & ~nw[i];
+        got |= nw[i];
+      }
+      still = rest != 0;
+      if (got) {
+        uint64_t sn[HW];
+#pragma unroll
+        for (int i = 0; i < HW; i++) sn[i] = alive[i] & ~(miss[i] & ~nw[i]);
+        store_words<HW>(a.seen + (int64_t)v * W + h * HW, sn);
+      }
+    }
+    // Arrivals anywhere in the vertex: its whole level record is written.
+    unsigned arr = 0;
+#pragma unroll
+    for (int i = 0; i < HW; i++) arr |= nw[i] != 0 ? 1u : 0u;
+    if constexpr (TPV > 1) {
+      arr |= xor_lane_u32<1>(arr);
+      still |= xor_lane_u32<1>(still);
+    }
+    if (light) {
+      if (arr) store_words<HW>(cur + (int64_t)v * W + h * HW, nw);
+#pragma unroll
+      for (int i = 0; i < HW; i++) {
+        c_lanes += (unsigned)__popcll(nw[i]);
+        const uint64_t x = nw[i] & ~s_alive[h * HW + i];
+        if (x) atomicOr(&s_alive[h * HW + i], (unsigned long long)x);
+      }
+      if (h == 0) {
+        if (arr) {
+          const int d = out_deg(a, v);
+          c_fe += (unsigned)d;
+          c_scout += (unsigned)(d > 0 ? d : 1);
+        }
+        if (still) c_ue += (unsigned)deg;
+      }
+    }
+    // Frontier and done bitmap words of the pass's vertices (heavy ones:
+    // frontier bit clear, done bit as before; the finalize pass adds theirs).
+    const uint64_t fm = __ballot(light && arr && h == 0);
+    const uint64_t dm = __ballot(h == 0 && (done_before || (light && !still)));
+    c_fv += (unsigned long long)__popcll(fm);
+    c_uv += (unsigned long long)__popcll(__ballot(h == 0 && light && still));
+    if (lane == 0) {
+      if constexpr (TPV == 1) {
+        fbo[base >> 5] = (uint32_t)fm;
+        fbo[(base >> 5) + 1] = (uint32_t)(fm >> 32);
+        a.done[base >> 5] = (uint32_t)dm;
+        a.done[(base >> 5) + 1] = (uint32_t)(dm >> 32);
+      } else {
+        fbo[base >> 5] = even_bits(fm);
+        a.done[base >> 5] = even_bits(dm);
+      }
+    }
   }
+  // Workgroup totals, one set of atomics per workgroup.
+  const unsigned long long r[6] = {c_fv, wave_sum_u64(c_fe), wave_sum_u64(c_scout),
+                                   wave_sum_u64(c_lanes), c_uv, wave_sum_u64(c_ue)};
+  if (lane == 0)
+    for (int i = 0; i < 6; i++) s_cnt[threadIdx.x / kWave][i] = r[i];
+  __syncthreads();
+  LevelCounters* c = a.ring + (lv % kRing);
+  if (threadIdx.x == 0) {
+    unsigned long long t[6] = {};
+    for (int w = 0; w < kBlock / kWave; w++)
+      for (int i = 0; i < 6; i++) t[i] += s_cnt[w][i];
+    if (t[0]) {
+      atomicAdd(&c->fv, t[0]);
+      atomicAdd(&c->fe, t[1]);
+      atomicAdd(&c->scout, t[2]);
+      atomicAdd(&c->flanes, t[3]);
+    }
+    if (t[4]) {
+      atomicAdd(&c->uv, t[4]);
+      atomicAdd(&c->ue, t[5]);
+    }
+  }
+  if (threadIdx.x < W && s_alive[threadIdx.x])
+    atomicOr((unsigned long long*)&c->alive[threadIdx.x], (unsigned long long)s_alive[threadIdx.x]);
 }
 
 // Expansion of one step, one kernel per mode (the device scheduler picks
@@ -568,7 +705,7 @@ __global__ void __launch_bounds__(kBlock) ms_topdown_kernel(MsArgs a) {
     uint64_t any = 0;
 #pragma unroll
     for (int k = 0; k < HW; k++) any |= f[k];
-    if constexpr (TPV > 1) any |= shfl_xor_u64(any, 1);
+    if constexpr (TPV > 1) any |= xor_lane_u64<1>(any);
     if (!any) continue;
     const int e = a.oc_end[c];
     for (int k = a.oc_beg[c] + lane / TPV; k < e; k += kWave / TPV) {
@@ -668,11 +805,11 @@ __global__ void __launch_bounds__(kBlock) ms_pull_kernel(MsArgs a) {
     }
   } else if constexpr (PV == 0) {
     // Gathers in flight per lane: fewer for wide records (registers).
-    pull_light_lanes<W, (W > 4 ? 2 : 4), false>(a, prev, fb, alive);
+    pull_light_fused<W, (W > 4 ? 2 : 4), false>(a, lv, prev, fb, alive);
   } else if constexpr (PV == 2) {
-    pull_light_lanes<W, (W > 4 ? 4 : 8), false>(a, prev, fb, alive);
+    pull_light_fused<W, (W > 4 ? 4 : 8), false>(a, lv, prev, fb, alive);
   } else {
-    pull_light_lanes<W, (W > 4 ? 4 : 8), true>(a, prev, fb, alive);
+    pull_light_fused<W, (W > 4 ? 4 : 8), true>(a, lv, prev, fb, alive);
   }
   // Heavy vertices: their in-edge lists are cut into chunks of at most
   // knobs.pull_chunk edges, one wave per chunk (lane pairs per edge at
@@ -693,8 +830,8 @@ __global__ void __launch_bounds__(kBlock) ms_pull_kernel(MsArgs a) {
       found = found && (nx[k] & miss[k]) == miss[k];  // other chunks found them all
     }
     if constexpr (TPV > 1) {
-      any |= shfl_xor_u64(any, 1);
-      found = __shfl_xor((int)found, 1) && found;
+      any |= xor_lane_u64<1>(any);
+      found = (xor_lane_u32<1>(found ? 1u : 0u) != 0) && found;
     }
     if (!any || found) continue;
     const int he = a.hc_end[c];
@@ -746,6 +883,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
 #pragma unroll
   for (int i = 0; i < WPV; i++) alive_prev[i] = al[sub * WPT + i];
   MsAcc<W, WPV> acc;
+  if (st.mode == DAWN_MODE_PULL && pull_folds_light<W>(a)) return;  // ms_fold_heavy_kernel
   constexpr int64_t kTile = (int64_t)kBlock / TPV * V;
   for (int64_t b0 = (int64_t)blockIdx.x * kTile; b0 < a.npad; b0 += (int64_t)gridDim.x * kTile) {
     const int64_t v0 = b0 + (int64_t)(threadIdx.x / TPV) * V;
@@ -849,6 +987,103 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   acc.flush(c_out, sub);
 }
 
+// Fold of a pull step's heavy vertices (in-degree >= knobs.heavy_degree),
+// whose chunks merged their arrivals in next[] (the pull kernel folded the
+// light ones): one group of TPV threads per chunk entry, acting on the
+// first chunk of each vertex not yet done.  Same arithmetic as the
+// finalize pass; bitmap bits are OR-ed into the words the pull wrote.
+template <int W>
+__global__ void __launch_bounds__(kBlock) ms_fold_heavy_kernel(MsArgs a) {
+  constexpr int WPT = W > 8 ? 8 : 4;
+  constexpr int TPV = W > WPT ? W / WPT : 1;
+  constexpr int WPV = W < WPT ? W : WPT;
+  const SchedState st = *a.st;
+  if (st.done || st.mode != DAWN_MODE_PULL || !pull_folds_light<W>(a)) return;
+  const int lv = st.step;
+  uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
+  uint32_t* fb = a.fbits + (int64_t)(lv % a.H) * a.fwords;
+  LevelCounters* c_out = a.ring + (lv % kRing);
+  const int lane = lane_id();
+  const int sub = lane % TPV;
+  uint64_t alive_prev[WPV];
+  const unsigned long long* al = a.ring[(lv - 1) % kRing].alive;
+#pragma unroll
+  for (int i = 0; i < WPV; i++) alive_prev[i] = al[sub * WPT + i];
+  MsAcc<W, WPV> acc;
+  // Pull step: the pull kernel folded the light vertices.  Fold the heavy
+  // ones (merged by their chunks in next[]): one group of TPV threads per
+  // chunk entry, acting on the first chunk of each vertex not yet done.
+  const int64_t gstride = (int64_t)gridDim.x * (kBlock / TPV);
+  for (int64_t cb = (int64_t)blockIdx.x * (kBlock / TPV) + (threadIdx.x / kWave) * (kWave / TPV);
+       cb < a.num_hc; cb += gstride) {  // wave-uniform
+    const int64_t c = cb + lane / TPV;
+    int32_t v = 0;
+    bool work = false;
+    if (c < a.num_hc) {
+      v = a.hc_v[c];
+      work = (c == 0 || a.hc_v[c - 1] != v) && !((a.done[v >> 5] >> (v & 31)) & 1u);
+    }
+    uint64_t nx[WPV], sv[WPV], nw[WPV];
+#pragma unroll
+    for (int i = 0; i < WPV; i++) nx[i] = sv[i] = 0;
+    if (work) {
+      load_words<WPV>(a.next + (int64_t)v * W + sub * WPV, nx);
+      load_words<WPV>(a.seen + (int64_t)v * W + sub * WPV, sv);
+    }
+    uint64_t any = 0, unv = 0, had = 0;
+    unsigned pc = 0;
+#pragma unroll
+    for (int i = 0; i < WPV; i++) {
+      nw[i] = nx[i] & ~sv[i];
+      any |= nw[i];
+      had |= nx[i];
+      unv |= ~(sv[i] | nw[i]) & alive_prev[i];
+      pc += (unsigned)__popcll(nw[i]);
+    }
+    const uint64_t own = any;
+    if constexpr (TPV > 1) {  // every lane of the wave is here
+      any = group_or_u64<TPV>(any);
+      unv = group_or_u64<TPV>(unv);
+      pc = group_sum_u32<TPV>(pc);
+    }
+    if (work) {
+      if (any) store_words<WPV>(cur + (int64_t)v * W + sub * WPV, nw);
+      if (own) {
+        uint64_t sn[WPV];
+#pragma unroll
+        for (int i = 0; i < WPV; i++) sn[i] = sv[i] | nw[i];
+        store_words<WPV>(a.seen + (int64_t)v * W + sub * WPV, sn);
+      }
+      if (had) {
+        uint64_t z[WPV];
+#pragma unroll
+        for (int i = 0; i < WPV; i++) z[i] = 0;
+        store_words<WPV>(a.next + (int64_t)v * W + sub * WPV, z);
+      }
+#pragma unroll
+      for (int i = 0; i < WPV; i++) acc.alive[i] |= nw[i];
+      if (sub == 0) {
+        const uint32_t bit = 1u << (v & 31);
+        if (any) {
+          atomicOr(&fb[v >> 5], bit);
+          const int d = out_deg(a, v);
+          acc.fv += 1;
+          acc.fe += (unsigned)d;
+          acc.scout += (unsigned)(d > 0 ? d : 1);
+          acc.lanes += pc;
+        }
+        if (unv) {
+          acc.uv += 1;
+          acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+        } else {
+          atomicOr(&a.done[v >> 5], bit);
+        }
+      }
+    }
+  }
+  acc.flush(c_out, sub);
+}
+
 // Builds the frontier queue of the previous level from its bitmap, only
 // for a PUSH step: one entry per kPushChunk out-edges of each frontier
 // vertex, so the push step's work per entry is bounded whatever the degree.
@@ -881,7 +1116,10 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
 
 // Turns levels [lo, hi) of the history into distances.  from_state: lo and
 // hi come from the batch / scheduler state and the pass runs only once the
-// traversal is done (then also writes INF for lanes that never arrived).
+// traversal is done.  The first window (lo == 0) writes every entry of the
+// rows, INF where no level so far holds the lane; later windows (long
+// traversals, flushed before the history ring wraps) overwrite the entries
+// of the lanes that arrive in them.  So the pass never reads seen[].
 // Wave w of a workgroup handles word w % W of a 256-vertex tile (4 vertices
 // per lane), so the 64 rows of one word are written with 16-byte stores,
 // 1 KB per wave per row, and the W waves of a tile share its cache lines.
@@ -894,13 +1132,13 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   constexpr int TPB = W < 4 ? 4 / W : 1;  // vertex tiles per workgroup
   constexpr int BPT = W > 4 ? W / 4 : 1;  // workgroups per tile (grid is a multiple)
   __shared__ unsigned long long s_sum[64 * W], s_reach[64 * W], s_edges[64 * W];
-  int lo = lo_arg, hi = hi_arg, fin = final_arg;
+  int lo = lo_arg, hi = hi_arg;
+  (void)final_arg;
   if (from_state) {
     const SchedState st = *a.st;
     if (!st.done) return;
     lo = a.batch->lo;
     hi = st.levels;
-    fin = 1;
   }
   const int nl = a.batch->nlanes;
   int32_t* const dist = a.batch->dist;
@@ -927,17 +1165,17 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   const int64_t tile0 = W > 4 ? (int64_t)(xr / BPT) * kXcds + blockIdx.x % kXcds
                               : (int64_t)blockIdx.x * TPB + tile;
   const int rows = min(64, nl - 64 * wi);  // lanes of this word in the batch
-  // With one flush of every level (lo == 0, final) every (lane, vertex) is
-  // written: level or INF.
-  const bool vec_ok = dist && (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0) && lo == 0 && fin;
+  // The first window writes every (lane, vertex): 16-byte row stores when
+  // the rows are aligned.
+  const bool first = lo == 0;
+  const bool vec_ok = dist && (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0) && first;
   const int64_t wstride = (int64_t)(gridDim.x / BPT) * TPB * 256;
   for (int64_t base = tile0 * 256; base < a.n; base += wstride) {
     const int64_t v0 = base + 4 * lane;
-    uint64_t F[4], S[4][kSlices], sv[4];
+    uint64_t F[4], S[4][kSlices];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       F[j] = 0;
-      sv[j] = 0;
 #pragma unroll
       for (int k = 0; k < kSlices; k++) S[j][k] = 0;
     }
@@ -950,18 +1188,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       const int64_t v = v0 + j;
       q[j] = ident || v >= a.n ? v : (int64_t)a.perm[v];
     }
-    // seen decides which unreached rows get INF in partial writes; whole
-    // 16-byte row stores (vec_ok) write every entry and do not need it.
     if (v0 + 3 < a.n && ident) {
-      if (vec_ok) {
-      } else if constexpr (W == 1) {
-        const ulonglong2* sp = reinterpret_cast<const ulonglong2*>(a.seen + v0);
-        const ulonglong2 s01 = sp[0], s23 = sp[1];
-        sv[0] = s01.x; sv[1] = s01.y; sv[2] = s23.x; sv[3] = s23.y;
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; j++) sv[j] = a.seen[(v0 + j) * W + wi];
-      }
       for (int l = lo; l < hi; l++) {
         // Level words are valid only for the level's frontier vertices.
         const unsigned fm = (a.fbits[(int64_t)(l % a.H) * a.fwords + (v0 >> 5)] >> (v0 & 31)) & 15u;
@@ -989,10 +1216,6 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         }
       }
     } else if (v0 + 3 < a.n) {
-      if (!vec_ok) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) sv[j] = a.seen[q[j] * W + wi];
-      }
       // Outside the hub range the four positions are consecutive: their
       // bitmap bits come from one (or two adjacent) words.
       const bool run = q[1] == q[0] + 1 && q[2] == q[0] + 2 && q[3] == q[0] + 3;
@@ -1025,7 +1248,6 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       for (int j = 0; j < 4; j++) {
         if (v0 + j >= a.n) continue;
         const int64_t v = q[j];
-        sv[j] = a.seen[v * W + wi];
         for (int l = lo; l < hi; l++) {
           if (!((a.fbits[(int64_t)(l % a.H) * a.fwords + (v >> 5)] >> (v & 31)) & 1u)) continue;
           const uint64_t w = a.hist[(int64_t)(l % a.H) * a.npad * W + v * W + wi];
@@ -1044,11 +1266,10 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       // are zero, so all kSlices are summed without a test.
 #pragma unroll
       for (int h = 0; h < 2; h++) {
-        uint32_t Sh[4][kSlices], Fh[4], Vh[4];
+        uint32_t Sh[4][kSlices], Fh[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           Fh[j] = (uint32_t)(F[j] >> (32 * h));
-          Vh[j] = (uint32_t)(sv[j] >> (32 * h));
 #pragma unroll
           for (int k = 0; k < kSlices; k++) Sh[j][k] = (uint32_t)(S[j][k] >> (32 * h));
         }
@@ -1064,7 +1285,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
               if (k < K) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
             const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
             d[j] = in ? lo + (int32_t)x : kInf;
-            wr[j] = in || (fin && __builtin_amdgcn_ubfe(Vh[j], bb, 1) == 0);
+            wr[j] = in || first;
           }
           int32_t* row = dist + (int64_t)(64 * wi + 32 * h + bb) * stride + v0;
           if (full) {
@@ -1292,6 +1513,7 @@ void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t 
   ms_topdown_kernel<W><<<grid, kBlock, 0, s>>>(a);
   launch_pull<W>(a, grid, s);
   ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
+  ms_fold_heavy_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
 }
 
 // Runs the batches of an unweighted multi-source traversal, 64·W sources
@@ -1639,7 +1861,10 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
           ms_topdown_kernel<W><<<grid, kBlock, 0, s>>>(a);
         }
       }));
-      DAWN_TRY(timed(6, [&] { ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a); }));
+      DAWN_TRY(timed(6, [&] {
+        ms_finalize_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
+        ms_fold_heavy_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
+      }));
       if (st.step - lo >= a.H - 2) {  // flush before the history ring wraps
         DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
         lo = st.step;

@@ -229,6 +229,46 @@ def test_lane_words_exact(orc, words):
     assert len(lv) == -(-700 // (64 * words)) and (lv > 0).all()
 
 
+@pytest.mark.parametrize("words,variant", [(1, 1), (4, 0), (4, 1), (4, 2), (4, 3), (16, 0), (16, 2), (16, 3)])
+def test_pull_variants_exact(orc, words, variant):
+    """Every light-pull variant (variant 1 leaves the fold of every vertex to
+    the finalize pass; the others fold light vertices in the pull kernel and
+    heavy ones in a separate pass), with heavy chunks present: rows and sums
+    equal the oracle's under the pull-only and fixed schedules."""
+    log(f"case: RMAT scale 13 ef 16 seed 21, 600 sources, lane_words={words}, pull_variant={variant}")
+    h = dawn.HostGraph.rmat(13, 16, 21)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=words, pull_variant=variant, heavy_degree=16, pull_chunk=64)
+    srcs = dawn.gen_sources(h.n, 600, 21)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    for p in ("pull", "fixed", "learned"):
+        dist, sums = g.mssp(srcs, policy=p)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+
+
+@pytest.mark.parametrize("words,count", [(1, 50), (8, 300), (16, 1100)])
+def test_long_traversals_flush_windows(orc, words, count):
+    """Traversals deeper than one captured chunk (a 40x40 grid: up to 78
+    levels) flush the level history in several windows; every row entry
+    equals the oracle's, and the grid's unit distances are Manhattan."""
+    log(f"case: grid 40x40 no shortcuts, {count} sources, lane_words={words}")
+    side = 40
+    h = dawn.HostGraph.grid(side, 0.0, 3)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=words)
+    srcs = dawn.gen_sources(h.n, count, 3)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    ys, xs = np.divmod(np.arange(h.n), side)
+    for i in range(0, count, max(1, count // 7)):
+        sy, sx = divmod(int(srcs[i]), side)
+        assert np.array_equal(odist[i], np.abs(ys - sy) + np.abs(xs - sx))
+    for p in ("fixed", "pull", "push"):
+        dist, sums = g.mssp(srcs, policy=p)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+
+
 @pytest.mark.parametrize("count", [1, 9, 17, 33, 70])
 def test_weighted_batched_mssp_exact(orc, count):
     """Batched min-plus (8, 16, 32, 64 lanes; a partial second batch at 70):
