@@ -168,6 +168,15 @@ __device__ __forceinline__ uint64_t lanes_below_mask() {
   return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Orders the LDS accesses of one wave: what its lanes wrote before is seen
+// by its lanes after (LDS executes a wave's operations in order; this keeps
+// the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave-aggregated append: lanes with pred get consecutive slots of *counter.
 // Must be reached by every lane that may append (ballot over active lanes).
 __device__ __forceinline__ int wave_append(unsigned int* counter, bool pred) {

@@ -725,6 +725,7 @@ __global__ void __launch_bounds__(kBlock) ms_topdown_kernel(MsArgs a) {
 template <int W, int PV>
 __global__ void __launch_bounds__(kBlock) ms_pull_kernel(MsArgs a) {
   MS_STEP_PROLOGUE(DAWN_MODE_PULL);
+  __shared__ int s_hits[TPV > 1 ? kBlock : 1];  // frontier in-neighbours of a heavy chunk round
   // Only lanes alive in the previous level can arrive now: a vertex looks
   // for miss = (lanes it lacks) & (live lanes) and stops once all are found.
   // alive: the live lanes of this lane's HW words.
@@ -836,13 +837,45 @@ __global__ void __launch_bounds__(kBlock) ms_pull_kernel(MsArgs a) {
     if (!any || found) continue;
     const int he = a.hc_end[c];
     uint64_t part[HW] = {};
-    for (int k = a.hc_beg[c] + lane / TPV; k < he; k += kWave / TPV) {
-      const int32_t u = a.in_col[k];
-      if ((fb[u >> 5] >> (u & 31)) & 1u) {
-        uint64_t f[HW];
-        load_words<HW>(prev + (int64_t)u * W + h * HW, f);
+    if constexpr (TPV == 1) {
+      for (int k = a.hc_beg[c] + lane; k < he; k += kWave) {
+        const int32_t u = a.in_col[k];
+        if ((fb[u >> 5] >> (u & 31)) & 1u) {
+          uint64_t f[HW];
+          load_words<HW>(prev + (int64_t)u * W, f);
 #pragma unroll
-        for (int t = 0; t < HW; t++) part[t] |= f[t];
+          for (int t = 0; t < HW; t++) part[t] |= f[t];
+        }
+      }
+    } else {
+      // Split records: every lane probes one in-edge (64 per round), the
+      // in-neighbours found in the frontier are listed in LDS, and lane
+      // pairs gather the listed records (two halves of one line each).
+      // Sparse frontiers cost one probe per edge, not one per lane pair.
+      int* hits = s_hits + (threadIdx.x / kWave) * kWave;
+      for (int k0 = a.hc_beg[c]; k0 < he; k0 += kWave) {  // wave-uniform
+        const int k = k0 + lane;
+        int32_t u = 0;
+        bool hit = false;
+        if (k < he) {
+          u = a.in_col[k];
+          hit = (fb[u >> 5] >> (u & 31)) & 1u;
+        }
+        const uint64_t m = __ballot(hit);
+        if (!m) continue;
+        if (hit) hits[__popcll(m & lanes_below_mask())] = u;
+        wave_lds_sync();
+        const int nh = __popcll(m);
+        for (int b = 0; b < nh; b += kWave / TPV) {
+          const int i = b + lane / TPV;
+          if (i < nh) {
+            uint64_t f[HW];
+            load_words<HW>(prev + (int64_t)hits[i] * W + h * HW, f);
+#pragma unroll
+            for (int t = 0; t < HW; t++) part[t] |= f[t];
+          }
+        }
+        wave_lds_sync();  // the list is rewritten by the next round
       }
     }
 #pragma unroll
