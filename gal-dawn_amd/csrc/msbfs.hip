@@ -452,7 +452,9 @@ __device__ __forceinline__ void pull_light_fused(const MsArgs& a, int lv, const 
       uint64_t any = 0;
 #pragma unroll
       for (int i = 0; i < HW; i++) any |= miss[i];
-      if (any && deg > 0) {
+      bool scan = any != 0;
+      if constexpr (TPV > 1) scan = (xor_lane_u32<1>(scan ? 1u : 0u) != 0) || scan;  // pair: both lanes here
+      if (scan && deg > 0) {
         const int32_t* nb = a.in_col + beg;
         uint64_t acc_w[HW] = {};
         auto complete = [&]() {
@@ -467,24 +469,64 @@ __device__ __forceinline__ void pull_light_fused(const MsArgs& a, int lv, const 
 #pragma unroll
           for (int i = 0; i < HW; i++) acc_w[i] |= f[i];
         };
-        int j = 0;
-        for (; j + U <= deg; j += U) {
-          int32_t u[U];
-          uint32_t w[U];
+        if constexpr (TPV == 1) {
+          int j = 0;
+          for (; j + U <= deg; j += U) {
+            int32_t u[U];
+            uint32_t w[U];
 #pragma unroll
-          for (int t = 0; t < U; t++) u[t] = NT ? __builtin_nontemporal_load(nb + j + t) : nb[j + t];
+            for (int t = 0; t < U; t++) u[t] = NT ? __builtin_nontemporal_load(nb + j + t) : nb[j + t];
 #pragma unroll
-          for (int t = 0; t < U; t++) w[t] = fb[u[t] >> 5];
+            for (int t = 0; t < U; t++) w[t] = fb[u[t] >> 5];
 #pragma unroll
-          for (int t = 0; t < U; t++)
-            if ((w[t] >> (u[t] & 31)) & 1u) gather(u[t]);
-          if (complete()) break;
-        }
-        if (!complete())
-          for (; j < deg; j++) {
-            const int32_t u = NT ? __builtin_nontemporal_load(nb + j) : nb[j];
-            if ((fb[u >> 5] >> (u & 31)) & 1u) gather(u);
+            for (int t = 0; t < U; t++)
+              if ((w[t] >> (u[t] & 31)) & 1u) gather(u[t]);
+            if (complete()) break;
           }
+          if (!complete())
+            for (; j < deg; j++) {
+              const int32_t u = NT ? __builtin_nontemporal_load(nb + j) : nb[j];
+              if ((fb[u >> 5] >> (u & 31)) & 1u) gather(u);
+            }
+        } else {
+          // Lane pair: each lane probes every other edge (U per lane and
+          // round), the probes are exchanged, and both lanes gather the
+          // records that are in the frontier -- the even lane's first, so
+          // the two halves of a line are loaded by one instruction.  A lane
+          // whose half is complete still probes for its partner; the pair
+          // stops when both halves are.
+          bool mine = complete();
+          for (int j = 0; j < deg; j += 2 * U) {
+            int32_t r0[U], r1[U];
+            bool g0[U], g1[U];
+#pragma unroll
+            for (int t = 0; t < U; t++) {
+              const int e = j + 2 * t + h;
+              int32_t u = 0;
+              bool hit = false;
+              if (e < deg) {
+                u = NT ? __builtin_nontemporal_load(nb + e) : nb[e];
+                hit = (fb[u >> 5] >> (u & 31)) & 1u;
+              }
+              const int32_t pu = (int32_t)xor_lane_u32<1>((uint32_t)u);
+              const bool ph = xor_lane_u32<1>(hit ? 1u : 0u) != 0;
+              r0[t] = h ? pu : u;
+              g0[t] = h ? ph : hit;
+              r1[t] = h ? u : pu;
+              g1[t] = h ? hit : ph;
+            }
+            if (!mine) {
+#pragma unroll
+              for (int t = 0; t < U; t++) {
+                if (g0[t]) gather(r0[t]);
+                if (g1[t]) gather(r1[t]);
+              }
+              mine = complete();
+            }
+            const bool both = (xor_lane_u32<1>(mine ? 1u : 0u) != 0) && mine;  // same in both lanes
+            if (both) break;
+          }
+        }
 #pragma unroll
         for (int i = 0; i < HW; i++) nw[i] = acc_w[i] & miss[i];
       }
