@@ -91,23 +91,26 @@ def summarize_rocprof(out_dir, profiles_dir):
     write_factor = copy_bytes / (sum(per_dispatch(cw, "copy_probe")) / len(per_dispatch(cw, "copy_probe")))
 
     def traffic(kernel_sub):
-        # Dispatches that did work: the counter runs replay the same fixed
-        # schedule, so they are the ones with more than 1 MB of traffic.
+        # Dispatches that did work: longer than 20 us in the kernel trace.
+        # The counter runs replay the same batches in the same order (fixed
+        # schedule, one stream), so the same dispatch indices are used there.
+        dur = durations_us(kernel_sub)
         fx, wx = per_dispatch(fetch, kernel_sub), per_dispatch(write, kernel_sub)
-        act = [i for i in range(min(len(fx), len(wx))) if fx[i] + wx[i] > 1e6]
+        if not (len(dur) == len(fx) == len(wx)):
+            raise SystemExit(f"{kernel_sub}: {len(dur)} traced, {len(fx)}/{len(wx)} counted dispatches")
+        act = [i for i in range(len(dur)) if dur[i] > 20.0]
         if not act:
             return None
         f = sum(fx[i] for i in act) / len(act) * fetch_factor
         w = sum(wx[i] for i in act) / len(act) * write_factor
-        d = [x for x in durations_us(kernel_sub) if x > 20.0]
         return {"fetch_bytes": f, "write_bytes": w, "dispatches": len(act),
-                "avg_active_us": sum(d) / len(d) if d else None, "active_dispatches_traced": len(d)}
+                "avg_active_us": sum(dur[i] for i in act) / len(act)}
 
     pull = traffic("ms_pull_kernel")
     final = traffic("ms_final_kernel")
     os.makedirs(profiles_dir, exist_ok=True)
     summary = {
-        "workload": "RMAT scale 24 ef 16 seed 99, 4096 sources, fixed schedule (bench.py --quick-profile)",
+        "workload": "RMAT scale 24 ef 16 seed 99, 4096 sources, learned schedule, one batch per call (bench.py --quick-profile)",
         "fetch_factor": fetch_factor,
         "write_factor": write_factor,
         "bytes_per_launch": {"expand_pull": pull["fetch_bytes"] + pull["write_bytes"],
@@ -209,10 +212,16 @@ def main():
     log(f"learned network loaded: {learned_loaded}")
 
     if args.quick_profile:
-        # Fixed schedule: the same modes at the same steps every run, so the
-        # per-dispatch counters of the pull steps can be picked out.
-        _, sums, st = g.mssp(src, policy="fixed", want_dist=False, stats=True)
-        print(json.dumps({"quick_profile": True, "device_ms": st["device_ms"], "steps": st["steps"]}))
+        # Learned schedule (deterministic), one batch per call (one stream):
+        # the same kernels in the same order every run, so the dispatches of
+        # the counter runs line up with those of the kernel trace, and the
+        # steps are those of the profiling pass below.
+        dev_ms, steps = 0.0, 0
+        for b0 in range(0, len(src), 64 * W):
+            _, _, st = g.mssp(src[b0:b0 + 64 * W], policy="learned", want_dist=False, stats=True)
+            dev_ms += st["device_ms"]
+            steps += st["steps"]
+        print(json.dumps({"quick_profile": True, "device_ms": dev_ms, "steps": steps}))
         return
     copy_gbps = dawn.probe_copy(1 << 30, 5)
     log(f"plain copy: {copy_gbps:.0f} GB/s")
@@ -271,8 +280,8 @@ def main():
     log(f"schedule: {sched}")
 
     # Profiling pass (same kernels, one launch at a time with HIP events on
-    # the engine's stream) over one batch: per-kernel average durations.
-    prof, levels = g.mssp_profile(src[:64 * W], policy="learned")
+    # the engine's stream) over every batch: per-kernel average durations.
+    prof, levels = g.mssp_profile(src, policy="learned")
     kern = {k: {"total_ms": v[0], "launches": v[1], "avg_ms": v[0] / v[1] if v[1] else 0.0}
             for k, v in prof.items()}
     L = float(levels.mean())

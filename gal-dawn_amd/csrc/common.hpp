@@ -80,7 +80,7 @@ struct TuneKnobs {
   int32_t ss_chunk = 0;         // single-source steps per captured graph (0: auto)
   int32_t pull_variant = 0;     // light pull: 0 per lane 4 in flight, 1 edge-balanced,
                                 // 2 per lane 8 in flight, 3 as 2 with streaming column loads
-  int32_t lane_words = 8;       // batched engine: 64-lane words per vertex (1, 2, 4, 8, 16)
+  int32_t lane_words = 16;      // batched engine: 64-lane words per vertex (1, 2, 4, 8, 16)
   int32_t hub_order = 1;        // batched engine: hub-first vertex order (MsView)
 };
 

@@ -1957,12 +1957,23 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, i
   if (!g || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> guard(g->lock);
   DAWN_HIP(hipSetDevice(g->device));
-  switch (words_for(g, S)) {
-    case 1: return run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats);
-    case 2: return run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats);
-    case 4: return run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats);
-    case 8: return run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats);
-    default: return run_msbfs_w<16>(g, sources, S, dist, sums, reached, edges, policy, stats);
+  // Workspaces are sized by the batch width; if the device cannot hold them
+  // (they are allocated before any batch runs), narrower batches are tried.
+  for (int w = words_for(g, S);; w >>= 1) {
+    int rc;
+    switch (w) {
+      case 1: rc = run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
+      case 2: rc = run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
+      case 4: rc = run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
+      case 8: rc = run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
+      default: rc = run_msbfs_w<16>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
+    }
+    if (rc != DAWN_ERR_OUT_OF_MEMORY || w == 1) return rc;
+    ms_workspace_free(g->ms);
+    g->ms = nullptr;
+    ms_workspace_free(g->ms2);
+    g->ms2 = nullptr;
+    (void)hipGetLastError();
   }
 }
 

@@ -1,7 +1,7 @@
 """Train the step scheduler's MLP on per-step mode timings.
 
 1. Record: on each training graph the engine runs batches of 64*W sources
-   (W = the graph's lane_words knob, 4 by default: 256 sources); at
+   (W = the graph's lane_words knob, 16 by default: 1024 sources); at
    every step it times PUSH, PULL and BITMAP from the same state (HIP
    events, minimum of `repeats`) and stores the step's 8 features
    (dawn_sched_record).
