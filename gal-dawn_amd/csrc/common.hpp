@@ -313,6 +313,13 @@ __device__ __forceinline__ uint32_t group_sum_u32(uint32_t x) {
   else return x;
 }
 
+// Maximum over the wave through xor_lane exchanges (all lanes active).
+template <int J = 32>
+__device__ __forceinline__ int wave_max_xor(int x) {
+  if constexpr (J >= 1) return wave_max_xor<J / 2>(max(x, (int)xor_lane_u32<J>((uint32_t)x)));
+  else return x;
+}
+
 // One round of the 64x64 bit transpose: swaps bit J of the row index (the
 // lane) with bit J of the column index.  M: the low J bits of every 2J-bit
 // block.  Branch-free: the lane's own half is kept, the other half comes

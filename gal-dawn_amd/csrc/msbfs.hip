@@ -63,7 +63,8 @@ namespace dawn {
 constexpr int kMaxWords = 16;
 typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
 constexpr int kMaxLanes = 64 * kMaxWords;
-constexpr int kSlices = 5;       // bits of a relative level (2^5 >= level slots)
+// Bits of a level relative to the window start: 2^slices >= level slots.
+constexpr int level_slices(int W) { return W > 8 ? 4 : 5; }
 
 // Level slots of the history ring: 32, or 16 at W = 16 (128-byte records:
 // 16 slots are 34 GB at n = 2^24).
@@ -1189,6 +1190,10 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
   }
 }
 
+// Waves per SIMD the distance pass asks the compiler for: 4 (at most 128
+// VGPRs) where that needs no spills, else no constraint.
+constexpr int final_waves(int W) { return W == 1 || W == 16 ? 4 : 1; }
+
 // Turns levels [lo, hi) of the history into distances.  from_state: lo and
 // hi come from the batch / scheduler state and the pass runs only once the
 // traversal is done.  The first window (lo == 0) writes every entry of the
@@ -1202,8 +1207,10 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
 // ids x, x + 8, ... of the round-robin dispatch), so each record line is
 // fetched into one L2, not into W/4 of them.
 template <int W>
-__global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int from_state,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(final_waves(W)))) ms_final_kernel(MsArgs a, MsOut o, int from_state,
                                                           int lo_arg, int hi_arg, int final_arg) {
+  constexpr int KS = level_slices(W);
+  static_assert((1 << level_slices(W)) >= hist_slots(W), "window levels fit the slices");
   constexpr int TPB = W < 4 ? 4 / W : 1;  // vertex tiles per workgroup
   constexpr int BPT = W > 4 ? W / 4 : 1;  // workgroups per tile (grid is a multiple)
   __shared__ unsigned long long s_sum[64 * W], s_reach[64 * W], s_edges[64 * W];
@@ -1225,7 +1232,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   }
   __syncthreads();
   // Levels are stored relative to lo in K bit slices: S_k = OR of the
-  // level words whose (l - lo) has bit k set.  hi - lo <= H <= 2^kSlices.
+  // level words whose (l - lo) has bit k set.  hi - lo <= H <= 2^KS.
   int K = 0;
   while ((1 << K) < hi - lo) K++;
   const int lane = lane_id();
@@ -1247,21 +1254,21 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
   const int64_t wstride = (int64_t)(gridDim.x / BPT) * TPB * 256;
   for (int64_t base = tile0 * 256; base < a.n; base += wstride) {
     const int64_t v0 = base + 4 * lane;
-    uint64_t F[4], S[4][kSlices];
+    uint64_t F[4], S[4][KS];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       F[j] = 0;
 #pragma unroll
-      for (int k = 0; k < kSlices; k++) S[j][k] = 0;
+      for (int k = 0; k < KS; k++) S[j][k] = 0;
     }
     // Rows are in the original order; the records live at q[j], the hub-
     // first position of v0 + j (identity without a view).
     const bool ident = a.perm == nullptr;
-    int64_t q[4];
+    int32_t q[4];  // vertex ids < n < 2^31
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int64_t v = v0 + j;
-      q[j] = ident || v >= a.n ? v : (int64_t)a.perm[v];
+      const int32_t v = (int32_t)(v0 + j);
+      q[j] = ident || v >= a.n ? v : a.perm[v];
     }
     if (v0 + 3 < a.n && ident) {
       for (int l = lo; l < hi; l++) {
@@ -1286,7 +1293,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         for (int j = 0; j < 4; j++) {
           F[j] |= w[j];
 #pragma unroll
-          for (int k = 0; k < kSlices; k++)
+          for (int k = 0; k < KS; k++)
             if ((r >> k) & 1) S[j][k] |= w[j];
         }
       }
@@ -1294,7 +1301,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       // Outside the hub range the four positions are consecutive: their
       // bitmap bits come from one (or two adjacent) words.
       const bool run = q[1] == q[0] + 1 && q[2] == q[0] + 2 && q[3] == q[0] + 3;
-      const int64_t qw = q[0] >> 5;
+      const int32_t qw = q[0] >> 5;
       const int qb = (int)(q[0] & 31);
       for (int l = lo; l < hi; l++) {
         const uint32_t* fbl = a.fbits + (int64_t)(l % a.H) * a.fwords;
@@ -1312,10 +1319,10 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
         const int r = l - lo;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          const uint64_t w = ((fm >> j) & 1u) ? hb[q[j] * W + wi] : 0ull;
+          const uint64_t w = ((fm >> j) & 1u) ? hb[(int64_t)q[j] * W + wi] : 0ull;
           F[j] |= w;
 #pragma unroll
-          for (int k = 0; k < kSlices; k++)
+          for (int k = 0; k < KS; k++)
             if ((r >> k) & 1) S[j][k] |= w;
         }
       }
@@ -1329,7 +1336,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
           const int r = l - lo;
           F[j] |= w;
 #pragma unroll
-          for (int k = 0; k < kSlices; k++)
+          for (int k = 0; k < KS; k++)
             if ((r >> k) & 1) S[j][k] |= w;
         }
       }
@@ -1338,15 +1345,15 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
       const bool full = vec_ok && v0 + 3 < a.n;
       // Rows 0-31 of the word from the low halves, 32-63 from the high
       // halves: one bit-field extract per (vertex, slice).  Slices k >= K
-      // are zero, so all kSlices are summed without a test.
+      // are zero, so all KS are summed without a test.
 #pragma unroll
       for (int h = 0; h < 2; h++) {
-        uint32_t Sh[4][kSlices], Fh[4];
+        uint32_t Sh[4][KS], Fh[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           Fh[j] = (uint32_t)(F[j] >> (32 * h));
 #pragma unroll
-          for (int k = 0; k < kSlices; k++) Sh[j][k] = (uint32_t)(S[j][k] >> (32 * h));
+          for (int k = 0; k < KS; k++) Sh[j][k] = (uint32_t)(S[j][k] >> (32 * h));
         }
         const int bmax = min(32, rows - 32 * h);
         for (int bb = 0; bb < bmax; bb++) {
@@ -1356,7 +1363,7 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
           for (int j = 0; j < 4; j++) {
             uint32_t x = 0;
 #pragma unroll
-            for (int k = 0; k < kSlices; k++)
+            for (int k = 0; k < KS; k++)
               if (k < K) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
             const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
             d[j] = in ? lo + (int32_t)x : kInf;
@@ -1378,36 +1385,32 @@ __global__ void __launch_bounds__(kBlock) ms_final_kernel(MsArgs a, MsOut o, int
     // Per-lane sums: sum_b = lo * reach_b + sum_k 2^k * |{v : bit b of S_k}|,
     // counted with 64x64 bit transposes (lane b receives lane-b bits).
     unsigned long long sum = 0, reach = 0, edges = 0;
-    uint64_t tfs[4];
+    // edges_b = sum_i [bit i of tf] * deg_i, bit-sliced over the degree:
+    // sum_t 2^t * popcount(tf & ballot(bit t of deg)), over the degree bits
+    // of this wave's vertices only (hubs are rare).
+    int dg[4] = {0, 0, 0, 0};
+    int dbits = 0;
+    if (o.want_edges) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) dg[j] = v0 + j < a.n ? out_deg(a, q[j]) : 0;
+      const int dmax = wave_max_xor(max(max(dg[0], dg[1]), max(dg[2], dg[3])));
+      dbits = dmax > 0 ? 32 - __builtin_clz((unsigned)dmax) : 0;
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint64_t tf = wave_transpose64(F[j]);
-      tfs[j] = tf;
       const unsigned long long rc = (unsigned long long)__popcll(tf);
       reach += rc;
       sum += rc * (unsigned long long)lo;
+      for (int t = 0; t < dbits; t++) {
+        const uint64_t dm = __ballot((dg[j] >> t) & 1);
+        edges += (unsigned long long)__popcll(tf & dm) << t;
+      }
 #pragma unroll
-      for (int k = 0; k < kSlices; k++) {
+      for (int k = 0; k < KS; k++) {
         if (k < K) {
           const uint64_t t = wave_transpose64(S[j][k]);
           sum += (unsigned long long)__popcll(t) << k;
-        }
-      }
-    }
-    if (o.want_edges) {
-      // edges_b = sum_i [bit i of tf] * deg_i, bit-sliced over the degree:
-      // sum_t 2^t * popcount(tf & ballot(bit t of deg)).
-      int dg[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) dg[j] = v0 + j < a.n ? out_deg(a, (int32_t)q[j]) : 0;
-      // Degree bits of this wave's vertices only (hubs are rare).
-      const int dmax = wave_max_i32(max(max(dg[0], dg[1]), max(dg[2], dg[3])));
-      const int dbits = dmax > 0 ? 32 - __builtin_clz((unsigned)dmax) : 0;
-      for (int t = 0; t < dbits; t++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const uint64_t dm = __ballot((dg[j] >> t) & 1);
-          edges += (unsigned long long)__popcll(tfs[j] & dm) << t;
         }
       }
     }
