@@ -1356,28 +1356,38 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fin
           for (int k = 0; k < KS; k++) Sh[j][k] = (uint32_t)(S[j][k] >> (32 * h));
         }
         const int bmax = min(32, rows - 32 * h);
-        for (int bb = 0; bb < bmax; bb++) {
-          int32_t d[4];
-          bool wr[4];
+        int32_t* const row0 = dist + (int64_t)(64 * wi + 32 * h) * stride + v0;
+        // Level of vertex j in row bb relative to lo (slices k >= K are zero).
+        auto rel = [&](int j, int bb) {
+          uint32_t x = 0;
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            uint32_t x = 0;
+          for (int k = 0; k < KS; k++) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
+          return (int32_t)x;
+        };
+        if (full && bmax == 32) {
+          // Whole 16-byte row segments, no per-row branch: the level or INF by a
+          // bit select on the sign-extended "reached" bit; streaming stores
+          // (rows are not read back here).
+#pragma unroll 1
+          for (int bb = 0; bb < 32; bb++) {
+            v4i32 val;
 #pragma unroll
-            for (int k = 0; k < KS; k++)
-              if (k < K) x |= __builtin_amdgcn_ubfe(Sh[j][k], bb, 1) << k;
-            const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
-            d[j] = in ? lo + (int32_t)x : kInf;
-            wr[j] = in || first;
+            for (int j = 0; j < 4; j++) {
+              const int32_t m = __builtin_amdgcn_sbfe((int32_t)Fh[j], bb, 1);
+              val[j] = (m & (lo + rel(j, bb))) | (~m & kInf);
+            }
+            __builtin_nontemporal_store(val, reinterpret_cast<v4i32*>(row0 + (int64_t)bb * stride));
           }
-          int32_t* row = dist + (int64_t)(64 * wi + 32 * h + bb) * stride + v0;
-          if (full) {
-            // Rows are not read back here: streaming stores.
-            const v4i32 val = {d[0], d[1], d[2], d[3]};
-            __builtin_nontemporal_store(val, reinterpret_cast<v4i32*>(row));
-          } else {
+        } else {
+          for (int bb = 0; bb < bmax; bb++) {
+            int32_t* row = row0 + (int64_t)bb * stride;
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-              if (v0 + j < a.n && wr[j]) row[j] = d[j];
+            for (int j = 0; j < 4; j++) {
+              const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
+              const int32_t d = in ? lo + rel(j, bb) : kInf;
+              if (full) row[j] = d;
+              else if (v0 + j < a.n && (in || first)) row[j] = d;
+            }
           }
         }
       }
