@@ -1190,6 +1190,21 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
   }
 }
 
+// out = a + b on N-bit numbers held as bit slices (slice t: bit t of every
+// lane's number, one 64-bit mask per slice); out has N + 1 slices.
+template <int N>
+__device__ __forceinline__ void bitsliced_add(const uint64_t (&a)[N], const uint64_t (&b)[N],
+                                              uint64_t (&out)[N + 1]) {
+  uint64_t c = 0;
+#pragma unroll
+  for (int t = 0; t < N; t++) {
+    const uint64_t x = a[t] ^ b[t];
+    out[t] = x ^ c;
+    c = (a[t] & b[t]) | (c & x);
+  }
+  out[N] = c;
+}
+
 // Waves per SIMD the distance pass asks the compiler for: 4 (at most 128
 // VGPRs) where that needs no spills, else no constraint.
 constexpr int final_waves(int W) { return W == 1 || W == 16 ? 4 : 1; }
@@ -1416,12 +1431,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fin
         const uint64_t dm = __ballot((dg[j] >> t) & 1);
         edges += (unsigned long long)__popcll(tf & dm) << t;
       }
+    }
+    // The four vertices' relative levels per row, added bit-sliced in the
+    // thread (KS+2 bit slices of the row sums), then one transpose per slice.
+    {
+      uint64_t s01[KS + 1], s23[KS + 1], tot[KS + 2];
+      bitsliced_add<KS>(S[0], S[1], s01);
+      bitsliced_add<KS>(S[2], S[3], s23);
+      bitsliced_add<KS + 1>(s01, s23, tot);
 #pragma unroll
-      for (int k = 0; k < KS; k++) {
-        if (k < K) {
-          const uint64_t t = wave_transpose64(S[j][k]);
-          sum += (unsigned long long)__popcll(t) << k;
-        }
+      for (int t = 0; t < KS + 2; t++) {
+        const uint64_t m = wave_transpose64(tot[t]);
+        sum += (unsigned long long)__popcll(m) << t;
       }
     }
     const int gl = 64 * wi + lane;
