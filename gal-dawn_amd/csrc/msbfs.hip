@@ -1222,15 +1222,14 @@ constexpr int final_waves(int W) { return W == 1 || W == 16 ? 4 : 1; }
 // ids x, x + 8, ... of the round-robin dispatch), so each record line is
 // fetched into one L2, not into W/4 of them.
 template <int W>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(final_waves(W)))) ms_final_kernel(MsArgs a, MsOut o, int from_state,
-                                                          int lo_arg, int hi_arg, int final_arg) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(final_waves(W))))
+ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
   constexpr int KS = level_slices(W);
   static_assert((1 << level_slices(W)) >= hist_slots(W), "window levels fit the slices");
   constexpr int TPB = W < 4 ? 4 / W : 1;  // vertex tiles per workgroup
   constexpr int BPT = W > 4 ? W / 4 : 1;  // workgroups per tile (grid is a multiple)
   __shared__ unsigned long long s_sum[64 * W], s_reach[64 * W], s_edges[64 * W];
   int lo = lo_arg, hi = hi_arg;
-  (void)final_arg;
   if (from_state) {
     const SchedState st = *a.st;
     if (!st.done) return;
@@ -1666,7 +1665,7 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
       ms_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
       launch_step<W>(a, grid, fin_grid, s);
     }
-    ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 1, 0, 0, 1);
+    ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 1, 0, 0);
     DAWN_HIP(hipStreamEndCapture(s, &graph));
     hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -1700,7 +1699,7 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
       MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, want_edges};
       const int stp = ws->h_state->step;
       if (stp > lo) {
-        ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, stp, 0);
+        ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, stp);
         lo = stp;
       }
       for (int i = 0; i < kChunk; i++) {
@@ -1711,7 +1710,7 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
       DAWN_HIP(hipStreamSynchronize(s));
       if (ws->h_state->done) {
-        ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels, 1);
+        ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, ws->h_state->levels);
         break;
       }
       if (guard_steps > (int64_t)g->n + 2 * kChunk) return DAWN_ERR_HIP;
@@ -1955,7 +1954,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
       DAWN_HIP(hipMemcpy(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost));
       const SchedState st = *ws->h_state;
       if (st.done) {
-        DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.levels, 1); }));
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.levels); }));
         if (levels_out) levels_out[batch] = st.levels;
         break;
       }
@@ -1975,7 +1974,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
         ms_fold_heavy_kernel<W><<<fin_grid, kBlock, 0, s>>>(a);
       }));
       if (st.step - lo >= a.H - 2) {  // flush before the history ring wraps
-        DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step, 0); }));
+        DAWN_TRY(timed(7, [&] { ms_final_kernel<W><<<fgrid, kBlock, 0, s>>>(a, o, 0, lo, st.step); }));
         lo = st.step;
       }
     }
