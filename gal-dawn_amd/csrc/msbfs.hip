@@ -14,9 +14,11 @@
 // words each) wherever one lane would hold a whole record.
 //
 // Layout in HBM (per graph, reused across batches; v-major, W words each):
-//   seen   u64[npad][W]        lanes that have reached v
-//   next   u64[npad][W]        scatter target of the step (kept all-zero
-//                              between steps by the finalize pass)
+//   seen   u64[npad][W]        lanes that have reached v (after a pull step
+//                              only the live lanes: nothing reads the others)
+//   next   u64[npad][W]        scatter target of push / top-down steps and of
+//                              heavy pull chunks (zero between steps: every
+//                              fold clears what its step wrote)
 //   hist   u64[H][npad][W]     level history (H = 32 slots, 16 at W = 16):
 //                              slot l%H holds the lanes that
 //                              reached v exactly at level l (= frontier of l),
@@ -29,18 +31,20 @@
 //   q, qb  int32[qcap]         push queue: one entry per 256 out-edges of a
 //                              frontier vertex (vertex, first edge)
 //   dist   int32[64W][n]       output rows of the batch (caller or workspace)
-// Step modes (all produce the same next level in next[]):
+// Step modes (all produce the same next level; one kernel each, the device
+// scheduler's choice runs and the others return at once):
 //   PULL    every vertex missing a live lane ORs the previous level's words
 //           of its in-neighbours that are in the frontier bitmap (CSC):
-//           light vertices one lane each with early exit once all missing
-//           lanes are found (or edge-balanced over the workgroup); heavy
-//           vertices' in-edge lists in chunks, one wave per chunk, merged by
-//           atomicOr.
+//           light vertices one lane (pair) each with early exit once all
+//           missing lanes are found, folding their result in place (level
+//           record, seen, bitmap words, counters); heavy vertices' in-edge
+//           lists in chunks, one wave per chunk, merged by atomicOr into
+//           next[] and folded by a separate pass.
 //   PUSH    queue entries, workgroup-balanced expansion (LDS degree scan),
 //           atomicOr of (frontier & ~seen[v]) into next[v], word by word.
 //   BITMAP  dense scan of the frontier bitmap, same scatter as PUSH.
-//   Every step ends with a finalize pass that folds next into the level
-//   slot, seen, the frontier bitmap and the level counters; vertices marked
+//   PUSH and BITMAP steps end with a finalize pass that folds next into the
+//   level slot, seen, the bitmaps and the level counters; vertices marked
 //   done are skipped, and level words are written only for new arrivals.
 // After the last level one pass turns the history into int32 distances
 // (16-byte row stores) and per-lane sums/counts (64x64 bit transposes +
@@ -293,10 +297,11 @@ __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
   const MsBatch& b = *a.batch;
   const int64_t gs = (int64_t)gridDim.x * blockDim.x;
   const int64_t words = a.npad * W;
+  // next[] is not touched: it is zeroed when allocated and every step
+  // leaves it zero (the fold clears what the step wrote).
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < words; x += gs) {
     const int64_t v = x / W;
     a.seen[x] = v < a.n ? ~b.lanemask[x % W] : ~0ull;
-    a.next[x] = 0ull;
   }
   for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.fwords; w += gs) {
     a.fbits[w] = 0u;  // level-0 frontier bitmap
