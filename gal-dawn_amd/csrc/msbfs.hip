@@ -643,7 +643,7 @@ __global__ void __launch_bounds__(kBlock) ms_push_kernel(MsArgs a) {
   MS_STEP_PROLOGUE(DAWN_MODE_PUSH);
   __shared__ int s_off[kBlock];
   __shared__ int s_beg[kBlock];
-  __shared__ uint64_t s_f[kBlock * HW];
+  __shared__ uint64_t s_f[kBlock * W];
   __shared__ unsigned s_nz[kBlock];
   __shared__ int s_scan[kBlock / kWave];
   const int32_t* q_in = a.q;  // built by ms_compact_kernel for this step
@@ -652,54 +652,55 @@ __global__ void __launch_bounds__(kBlock) ms_push_kernel(MsArgs a) {
   for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < qn; c0 += (int64_t)gridDim.x * kBlock) {
     const int64_t i = c0 + threadIdx.x;
     int beg = 0, deg = 0;
-    int32_t u = 0;
+    unsigned nz = 0;  // nonzero words of the entry's frontier record
     if (i < qn) {
-      u = q_in[i];
+      const int32_t u = q_in[i];
       beg = qb_in[i];
       deg = min(a.row_ptr[u + 1] - beg, kPushChunk);
+      uint64_t f[HW];
+#pragma unroll
+      for (int p = 0; p < TPV; p++) {
+        load_words<HW>(prev + (int64_t)u * W + p * HW, f);
+#pragma unroll
+        for (int k = 0; k < HW; k++) {
+          s_f[threadIdx.x * W + p * HW + k] = f[k];
+          nz |= (f[k] != 0ull ? 1u : 0u) << (p * HW + k);
+        }
+      }
     }
+    if (!nz) deg = 0;  // an empty record reaches nobody
+    s_nz[threadIdx.x] = nz;
     int total;
     const int off = block_exclusive_scan(deg, s_scan, &total);
     s_off[threadIdx.x] = off;
     s_beg[threadIdx.x] = beg;
-    // One pass over the entries' edges per HW-word part of the records.
+    __syncthreads();
     // Frontier records of sparse levels hold few nonzero words: an edge
-    // touches seen/next only in the words its owner has (8-byte loads
-    // when at most half of them are set, whole-part loads otherwise), and
-    // owners with an empty part skip their edges.
-    for (int part = 0; part < TPV; part++) {
-      uint64_t f[HW] = {};
-      if (i < qn) load_words<HW>(prev + (int64_t)u * W + part * HW, f);
-      unsigned nz = 0;
+    // touches seen/next only in the words its owner has (8-byte accesses
+    // when at most half of them are set, whole-record loads otherwise).
+    for (int e = threadIdx.x; e < total; e += kBlock) {
+      const int j = owner_of(s_off, e);
+      unsigned msk = s_nz[j];
+      const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
+      const int64_t vw = (int64_t)v * W;
+      if (2 * __popc(msk) > W) {
 #pragma unroll
-      for (int k = 0; k < HW; k++) {
-        s_f[threadIdx.x * HW + k] = f[k];
-        nz |= (f[k] != 0ull ? 1u : 0u) << k;
-      }
-      s_nz[threadIdx.x] = nz;
-      __syncthreads();
-      for (int e = threadIdx.x; e < total; e += kBlock) {
-        const int j = owner_of(s_off, e);
-        unsigned msk = s_nz[j];
-        if (!msk) continue;
-        const int32_t v = a.col[s_beg[j] + (e - s_off[j])];
-        const int64_t vw = (int64_t)v * W + part * HW;
-        if (2 * __popc(msk) > HW) {
+        for (int p = 0; p < TPV; p++) {
           uint64_t sv[HW], nv[HW];
-          load_words<HW>(a.seen + vw, sv);
+          load_words<HW>(a.seen + vw + p * HW, sv);
 #pragma unroll
-          for (int k = 0; k < HW; k++) nv[k] = s_f[j * HW + k] & ~sv[k];
-          or_words<HW>(a.next + vw, nv);
-        } else {
-          for (; msk; msk &= msk - 1) {
-            const int k = __ffs(msk) - 1;
-            const uint64_t nv = s_f[j * HW + k] & ~a.seen[vw + k];
-            if (nv) atomicOr((unsigned long long*)&a.next[vw + k], (unsigned long long)nv);
-          }
+          for (int k = 0; k < HW; k++) nv[k] = s_f[j * W + p * HW + k] & ~sv[k];
+          or_words<HW>(a.next + vw + p * HW, nv);
+        }
+      } else {
+        for (; msk; msk &= msk - 1) {
+          const int k = __ffs(msk) - 1;
+          const uint64_t nv = s_f[j * W + k] & ~a.seen[vw + k];
+          if (nv) atomicOr((unsigned long long*)&a.next[vw + k], (unsigned long long)nv);
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
 }
 
