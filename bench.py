@@ -11,6 +11,7 @@ MSSP (oracle/_ref) timed on a bounded sample of the same workload.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24]
 """
 import argparse
+import importlib
 import json
 import os
 import sys
@@ -131,6 +132,88 @@ def summarize_rocprof(out_dir, profiles_dir):
     return summary, "\n".join(lines)
 
 
+def run_suite(dawn, orc, steps):
+    """The other configurations, one JSON line each (GPU device time, best of
+    `steps` runs after one warm-up; rows checked against the oracle on a
+    sample; the reference's CPU MSSP timed beside where it is correct)."""
+    wl = importlib.import_module("gal_dawn_amd.workloads")
+    threads = min(16, os.cpu_count() or 1)
+
+    def best(fn):
+        fn()
+        return min(fn()["device_ms"] for _ in range(steps))
+
+    # er65k: single-source BFS (source 0) and 65-source MSSP.
+    h, src = wl.er65k()
+    g = dawn.Graph(h)
+    bfs_ms = best(lambda: g.bfs(int(src[0]), stats=True)[1])
+    ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
+    dist, _ = g.mssp(src)
+    ok = bool(np.array_equal(dist, orc.mssp(h.row_ptr, h.col, None, src)[0]))
+    t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src, threads)
+    print(json.dumps({"config": "er65k: ER n=65,536 deg 8 seed 1, source 0 + 64", "bfs_ms": bfs_ms,
+                      "mssp_ms": ms_ms, "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
+                      "cpu_reference_sources_per_s": len(src) / t_cpu, "cpu_threads": threads,
+                      "rows_bit_exact": ok}), flush=True)
+    g.close()
+
+    # grid2048: weighted single-source SSSP (int32 min-plus) and 16-source MSSP.
+    h, src = wl.grid2048()
+    g = dawn.Graph(h)
+    sssp_ms = best(lambda: g.sssp(int(src[0]), stats=True)[1])
+    _, st = g.sssp(int(src[0]), stats=True)
+    ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
+    ms_steps = g.mssp(src, want_dist=False, stats=True)[2]["steps"]
+    singles_ms = sum(g.sssp(int(x), stats=True)[1]["device_ms"] for x in src)
+    ok = bool(np.array_equal(g.sssp(int(src[0])), orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[0]))))
+    rows, _ = g.mssp(src[:2])
+    ok = ok and all(np.array_equal(rows[i], orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[i])))
+                    for i in range(2))
+    print(json.dumps({"config": "grid2048: 2048^2 grid + 1% long-range, int32 w in [1,1000], seed 2",
+                      "sssp_ms": sssp_ms, "sssp_steps": st["steps"], "mssp16_ms": ms_ms,
+                      "mssp16_steps": ms_steps, "sssp_x16_ms": singles_ms,
+                      "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
+                      "rows_bit_exact_vs_dijkstra": ok}), flush=True)
+    g.close()
+
+    # rmat20: 4096-source MSSP.
+    h, src = wl.rmat20()
+    g = dawn.Graph(h)
+    runs = [g.mssp(src, want_dist=False, stats=True)[2] for _ in range(steps + 1)][1:]
+    st = min(runs, key=lambda r: r["device_ms"])
+    chk = src[:8]
+    ok = bool(np.array_equal(g.mssp(chk)[0], orc.mssp(h.row_ptr, h.col, None, chk)[0]))
+    t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src[:2 * threads], threads)
+    print(json.dumps({"config": "rmat20: RMAT scale 20 ef 16 seed 3, 4096 sources",
+                      "mssp_ms": st["device_ms"], "gteps": st["edges_reached"] / st["device_ms"] / 1e6,
+                      "sources_per_s": len(src) / (st["device_ms"] / 1e3),
+                      "cpu_reference_sources_per_s": 2 * threads / t_cpu, "cpu_threads": threads,
+                      "rows_bit_exact_8": ok}), flush=True)
+    g.close()
+
+    # uneven512: 512 graphs x 8 sources in one persistent launch.
+    for weighted in (False, True):
+        hs, src, _ = wl.uneven(weighted=weighted)
+        gs = [dawn.Graph(x) for x in hs]
+        dawn.batch_mssp(gs, src, 8, want_dist=False)
+        t = min(dawn.batch_mssp(gs, src, 8, want_dist=False)[1]["device_ms"] for _ in range(steps))
+        rows, _ = dawn.batch_mssp(gs, src, 8)
+        ok = True
+        for i in range(0, 512, 97):
+            x = hs[i]
+            if weighted:
+                ref_rows = [orc.dijkstra(x.row_ptr, x.col, x.weight, int(s)) for s in src[8 * i: 8 * i + 8]]
+            else:
+                ref_rows = orc.mssp(x.row_ptr, x.col, None, src[8 * i: 8 * i + 8])[0]
+            ok = ok and all(np.array_equal(rows[i][k], ref_rows[k]) for k in range(8))
+        print(json.dumps({"config": f"uneven512: 512 graphs (1k-130k vertices), 8 sources each, "
+                                    f"{'int32 weights' if weighted else 'unweighted'}, seed 4",
+                          "batch_ms": t, "sources_per_s": 512 * 8 / (t / 1e3),
+                          "vertices": int(sum(x.n for x in hs)), "rows_bit_exact_sample": ok}), flush=True)
+        for x in gs:
+            x.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +237,9 @@ def main():
                     help="one learned-schedule pass only (for rocprofv3 runs)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the 1 GiB plain copy kernel (rocprofv3 traffic calibration)")
+    ap.add_argument("--suite", action="store_true",
+                    help="the other workloads (ER 65k, weighted grid 2048^2, RMAT-20, 512 uneven "
+                         "graphs), one JSON line each, instead of the headline line")
     args = ap.parse_args()
     if args.summarize_rocprof:
         summary, table = summarize_rocprof(args.summarize_rocprof, os.path.join(ROOT, "profiles"))
@@ -172,6 +258,11 @@ def main():
         sys.exit(1)
     if args.probe_only:
         print(json.dumps({"copy_gbps": dawn.probe_copy(1 << 30, 3), "bytes": 1 << 30}))
+        return
+    if args.suite:
+        if os.path.exists(TUNE_PATH):
+            dawn.tune_load(TUNE_PATH)
+        run_suite(dawn, __graft_entry__.load_oracle(), max(1, args.steps))
         return
 
     t0 = time.time()

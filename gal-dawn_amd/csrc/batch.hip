@@ -403,7 +403,7 @@ extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
   (void)hipEventDestroy(e1);
   unsigned long long levels = 0;
   DAWN_HIP(hipMemcpy(&levels, dlev.p, sizeof levels, hipMemcpyDeviceToHost));
-  if (!dev_out)
+  if (dist && !dev_out)
     DAWN_HIP(hipMemcpy(dist, out, sizeof(int32_t) * (size_t)total, hipMemcpyDeviceToHost));
   if (stats) {
     stats->device_ms += ms;

@@ -18,12 +18,13 @@
 //                        vertex's distances of all lanes share cache lines
 //   act    u64[2][n]     lanes of v improved in the step (step parity)
 //   q      int32[2][n]   vertices with any improved lane (step parity)
-// Step (push, chaotic like GOVMP): one wave per frontier vertex u; the wave
-// is split into 64/L groups of L lanes, group g takes edges g, g+64/L, ...
-// of u, lane b of a group relaxes source lane b: dist[v][b] =
-// min(dist[v][b], dist[u][b] + w) for the lanes active at u.  The improved
-// lanes of a group are OR-ed into act_next[v] with one atomic; the first
-// one appends v to the next queue.  A K-step loop is captured once into a
+// Step (push, chaotic like GOVMP): a frontier vertex u of out-degree < 16
+// gets one lane, which relaxes dist[v][b] = min(dist[v][b], dist[u][b] + w)
+// for each lane b active at u and each edge in turn; a heavier u gets one
+// wave, split into 64/L groups of L lanes, group g taking edges g, g+64/L,
+// ... and lane b of a group relaxing source lane b.  Improved lanes are
+// OR-ed into act_next[v]; the first improvement appends v to the next
+// queue.  A K-step loop is captured once into a
 // hipGraph; every kernel reads the step and the done flag from device
 // state, so the graph is replayed until the frontier empties.
 #include <hip/hip_runtime.h>
@@ -40,6 +41,7 @@ namespace dawn {
 namespace {
 
 constexpr int kWChunk = 32;  // steps per captured graph
+constexpr int kWLight = 16;  // out-degree below which a queued vertex gets one lane, not a wave
 
 struct WState {
   int32_t step;      // parity of the current queue = step & 1
@@ -112,10 +114,59 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
   const int b = lane % L;
   const int grp = lane / L;
   const uint64_t gmask = L == 64 ? ~0ull : ((1ull << L) - 1);
+  // Light vertices (out-degree < kWLight): one lane per queued vertex, which
+  // walks its (active lane, edge) relaxations in turn -- on sparse graphs a
+  // vertex is usually active in few lanes, so a wave per vertex would leave
+  // most lanes idle.  The wave loops until its busiest lane is done, so the
+  // appends stay wave-aggregated.
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < qn;
+       base += (int64_t)gridDim.x * kBlock) {  // wave-uniform
+    const int64_t i = base + lane;
+    int32_t u = 0;
+    uint64_t m = 0;
+    int beg = 0, deg = 0;
+    if (i < qn) {
+      u = q_in[i];
+      beg = a.row_ptr[u];
+      deg = a.row_ptr[u + 1] - beg;
+      if (deg < kWLight) m = act_in[u];
+    }
+    const bool light = i < qn && deg < kWLight;
+    const int work = __popcll(m) * deg;
+    const int most = wave_max_i32(work);
+    int e = 0, bl = m ? __ffsll((unsigned long long)m) - 1 : 0;
+    int32_t du = m ? a.dist[(int64_t)u * L + bl] : kInf;
+    for (int it = 0; it < most; it++) {
+      bool append = false;
+      int32_t v = -1;
+      if (it < work) {
+        v = a.col[beg + e];
+        if (du != kInf) {
+          const int32_t nd = du + a.w[beg + e];
+          int32_t* dv = a.dist + (int64_t)v * L + bl;
+          if (nd < *dv && nd < atomicMin(dv, nd))
+            append = atomicOr((unsigned long long*)&act_out[v], 1ull << bl) == 0ull;
+        }
+        if (++e == deg) {  // next active lane of u
+          e = 0;
+          m &= m - 1;
+          if (m) {
+            bl = __ffsll((unsigned long long)m) - 1;
+            du = a.dist[(int64_t)u * L + bl];
+          }
+        }
+      }
+      const int pos = wave_append(qlen_out, append);
+      if (append) q_out[pos] = v;
+    }
+    if (light) act_in[u] = 0ull;  // consumed
+  }
+  // Heavy vertices: one wave each, split into groups of L lanes.
   const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
   for (int64_t i = wave; i < qn; i += nwaves) {
     const int32_t u = q_in[i];
+    if (a.row_ptr[u + 1] - a.row_ptr[u] < kWLight) continue;
     const uint64_t mask = act_in[u];
     const bool active = (mask >> b) & 1ull;
     const int32_t du = active ? a.dist[(int64_t)u * L + b] : kInf;

@@ -183,7 +183,9 @@ int dawn_closeness(const dawn_graph* g, const int32_t* sources, int32_t count,
 /* Uneven batch: many independent graphs in one persistent launch.
  * graphs[i] are device graphs; sources are count_per_graph per graph,
  * concatenated.  dist_offsets[i] = offset of graph i's first row in dist
- * (rows of n_i); dist may be host or device.  Weighted graphs run min-plus. */
+ * (rows of n_i); dist may be host or device, or NULL (rows computed into a
+ * scratch buffer and dropped, e.g. for timing).  Weighted graphs run
+ * min-plus. */
 int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
                     const int32_t* sources, int32_t count_per_graph,
                     int32_t* dist, const int64_t* dist_offsets,
