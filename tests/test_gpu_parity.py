@@ -269,6 +269,24 @@ def test_long_traversals_flush_windows(orc, words, count):
         assert np.array_equal(sums, osums), p
 
 
+@pytest.mark.parametrize("n", [10, 100])
+def test_tiny_graph_wide_batch(orc, n):
+    """A path far smaller than one batch (1100 sources taken mod n, so every
+    vertex is the source of many lanes; padded vertex range): rows are the
+    closed form |i - s| under every schedule."""
+    log(f"case: path of {n} vertices, 1100 sources, lane_words=16")
+    u = np.arange(n - 1, dtype=np.int32)
+    h = dawn.HostGraph.from_edges(n, u, u + 1)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=16)
+    srcs = dawn.gen_sources(1 << 20, 1100, 5)
+    want = np.abs(np.arange(n)[None, :] - (srcs % n)[:, None]).astype(np.int32)
+    for p in ("fixed", "pull", "push", "bitmap"):
+        dist, sums = g.mssp(srcs, policy=p)
+        assert np.array_equal(dist, want), p
+        assert np.array_equal(sums, want.sum(axis=1)), p
+
+
 @pytest.mark.parametrize("count", [1, 9, 17, 33, 70])
 def test_weighted_batched_mssp_exact(orc, count):
     """Batched min-plus (8, 16, 32, 64 lanes; a partial second batch at 70):
