@@ -194,9 +194,12 @@ int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
 /* Profiling pass of dawn_mssp (unweighted): the same kernels launched one
  * at a time with HIP events around each launch on the graph's stream,
  * writing distance rows and statistics like a timed traversal.
- * kernel_ms[8] / launches[8] are indexed by DAWN_PROF_*; levels
- * (ceil(count/64) entries) receives the level count of each batch of 64·W
- * sources (W = lane words in use), then -1 for the unused entries. */
+ * kernel_ms[8] / launches[8] are indexed by DAWN_PROF_*: EXPAND_* is the
+ * step kernel of the chosen mode (a pull step includes its in-place fold of
+ * the light vertices), FINALIZE the fold that follows it (the full fold
+ * after push / top-down steps, the heavy-vertex fold after pull steps);
+ * levels (ceil(count/64) entries) receives the level count of each batch of
+ * 64·W sources (W = lane words in use), then -1 for the unused entries. */
 enum {
   DAWN_PROF_FILL = 0, DAWN_PROF_SEED = 1, DAWN_PROF_DECIDE = 2,
   DAWN_PROF_EXPAND_PUSH = 3, DAWN_PROF_EXPAND_PULL = 4,
