@@ -328,6 +328,30 @@ def test_weighted_batched_multigraph_and_isolated(orc):
     assert st["skipped_sources"] == int(skipped.sum()) == 2
 
 
+@pytest.mark.parametrize("count", [5, 40, 64])
+def test_weighted_batched_skewed_directed(orc, count):
+    """Batched min-plus on a skewed directed graph: light frontier vertices
+    (out-degree < 16, one lane each walking their active source lanes) and
+    heavy ones (a wave each) in the same steps; rows equal Dijkstra's."""
+    log(f"case: weighted directed RMAT scale 12 ef 8 seed 17, {count} sources")
+    h = dawn.HostGraph.rmat(12, 8, 17, weighted=True)
+    rows = np.repeat(np.arange(h.n, dtype=np.int32), np.diff(h.row_ptr))
+    keep = (rows + h.col) % 3 != 0  # drop a third of the arcs: asymmetric
+    rp, cc = _directed_csr(h.n, rows[keep], h.col[keep])
+    order = np.lexsort((h.col[keep], rows[keep]))
+    ww = h.weight[keep][order]
+    hd = dawn.HostGraph.from_csr(rp, cc, ww, directed=True)
+    g = dawn.Graph(hd)
+    deg = np.diff(rp)
+    assert (deg >= 16).any() and (deg < 16).any()
+    srcs = dawn.gen_sources(hd.n, count, 17)
+    srcs[0] = int(np.argmax(deg))
+    dist, sums = g.mssp(srcs)
+    for i in range(count):
+        want = orc.dijkstra(rp, cc, ww, int(srcs[i]))
+        assert np.array_equal(dist[i], want), i
+
+
 @pytest.mark.parametrize("directed", [False, True])
 def test_hub_order_view_exact(orc, directed):
     """Hub-first renumbering of the batched engine (graphs >= 65,536
