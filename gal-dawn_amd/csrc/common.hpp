@@ -104,6 +104,7 @@ struct dawn_graph {
   int device = 0;
   int32_t max_out_degree = 0;
   int32_t max_in_degree = 0;
+  int64_t zero_in = 0;            // vertices of in-degree 0 (no step ever reaches them)
   int32_t* row_ptr = nullptr;     // [n+1]
   int32_t* col = nullptr;         // [m]
   int32_t* weight = nullptr;      // [m] or null

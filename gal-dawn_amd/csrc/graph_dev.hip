@@ -200,6 +200,12 @@ int dawn_graph_create(const dawn_csr* csr, dawn_graph** out) {
     if (csr->weight && (rc = up(&g->in_weight, iw.data(), (size_t)m))) return fail(rc);
   }
   if (g->directed) g->h_in_row_ptr = in_rp_host;
+  {
+    const std::vector<int32_t>& ip = g->directed ? g->h_in_row_ptr : g->h_row_ptr;
+    int64_t z = 0;
+    for (int32_t v = 0; v < n; v++) z += ip[v + 1] == ip[v];
+    g->zero_in = z;
+  }
   (void)tuned_knobs_for(g, &g->knobs);
   if ((rc = build_chunks(g))) return fail(rc);
   *out = g;

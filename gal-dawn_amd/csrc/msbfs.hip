@@ -96,6 +96,7 @@ struct MsArgs {
   int32_t H;
   int64_t npad;
   int64_t m;
+  int64_t zero_in;     // vertices of in-degree 0: done from the start
   const int32_t* row_ptr;
   const int32_t* col;
   const int32_t* in_row_ptr;
@@ -303,9 +304,22 @@ __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
     const int64_t v = x / W;
     a.seen[x] = v < a.n ? ~b.lanemask[x % W] : ~0ull;
   }
+  // A vertex of in-degree 0 never receives a lane (a source has it from
+  // level 0 on): it starts done, so no step reads its seen words.  The
+  // scheduler adds these vertices back to its unvisited count.
   for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.fwords; w += gs) {
     a.fbits[w] = 0u;  // level-0 frontier bitmap
-    a.done[w] = 0u;
+    uint32_t d = 0u;
+    if (a.zero_in) {
+      const int64_t v0 = w * 32;
+      int32_t r = v0 < a.n ? a.in_row_ptr[v0] : 0;
+      for (int j = 0; j < 32 && v0 + j < a.n; j++) {
+        const int32_t r1 = a.in_row_ptr[v0 + j + 1];
+        d |= (r1 == r ? 1u : 0u) << j;
+        r = r1;
+      }
+    }
+    a.done[w] = d;
   }
   if (blockIdx.x == 0 && threadIdx.x < kRing) a.ring[threadIdx.x] = LevelCounters{};
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 64 * W; t += gs) {
@@ -357,7 +371,7 @@ __global__ void __launch_bounds__(kMaxLanes) ms_seed_kernel(MsArgs a) {
   if (first) {
     bool full = true;
     for (int k = 0; k < W; k++) full = full && a.seen[(int64_t)v * W + k] == ~0ull;
-    if (full) {
+    if (full && a.in_row_ptr[v + 1] > a.in_row_ptr[v]) {  // in-degree 0: in zero_in
       atomicAdd(&s_c[2], 1ull);
       atomicAdd(&s_c[3], (unsigned long long)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]));
     }
@@ -370,7 +384,7 @@ __global__ void __launch_bounds__(kMaxLanes) ms_seed_kernel(MsArgs a) {
     c.scout = s_c[1];
     c.flanes = (unsigned long long)nl;
     for (int k = 0; k < W; k++) c.alive[k] = s_alive[k];
-    c.uv = (unsigned long long)a.n - s_c[2];
+    c.uv = (unsigned long long)(a.n - a.zero_in) - s_c[2];
     c.ue = (unsigned long long)a.m - s_c[3];
     a.ring[0] = c;
     SchedState st{};
@@ -1524,6 +1538,7 @@ MsArgs make_args(dawn_graph* g, MsWorkspace* ws) {
   a.H = ws->slots;
   a.npad = ws->npad;
   a.m = g->m;
+  a.zero_in = g->zero_in;
   const MsView* vw = g->view;
   a.row_ptr = vw ? vw->row_ptr : g->row_ptr;
   a.col = vw ? vw->col : g->col;
@@ -1568,6 +1583,7 @@ SchedParams make_params(dawn_graph* g, int32_t policy) {
   p.max_degree = (float)g->max_out_degree;
   p.mlp = sched_mlp_device(g->device);
   p.mlp_norm = sched_norm_device(g->device);
+  p.uv_base = g->zero_in;
   return p;
 }
 

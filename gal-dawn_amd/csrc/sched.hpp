@@ -56,6 +56,8 @@ struct SchedParams {
   float max_degree;
   const float* mlp;      // device params (DAWN_SCHED_PARAMS) or null
   const float* mlp_norm; // device mean[8], inv_std[8] or null
+  long long uv_base;     // unvisited vertices the counters leave out (the
+                         // batched engine never folds vertices of in-degree 0)
 };
 
 struct SchedTrace {
@@ -76,7 +78,7 @@ __host__ __device__ inline void sched_features(const LevelCounters& c, int step,
                                                const SchedParams& p, float* x) {
   x[0] = log2p1((double)c.fv);
   x[1] = log2p1((double)c.fe);
-  x[2] = log2p1((double)c.uv);
+  x[2] = log2p1((double)c.uv + (double)p.uv_base);
   x[3] = log2p1((double)c.ue);
   x[4] = (float)step;
   x[5] = log2p1((double)p.n);
