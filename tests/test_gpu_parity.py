@@ -402,3 +402,32 @@ def _directed_csr(n, u, v):
     rp = np.zeros(n + 1, np.int64)
     np.add.at(rp, u + 1, 1)
     return np.cumsum(rp).astype(np.int32), v.astype(np.int32)
+
+
+@pytest.mark.parametrize("words", [1, 16])
+def test_zero_in_degree_sources_exact(orc, words):
+    """Vertices of in-degree 0 start done (no step reads them): sources of
+    in-degree 0, a batch whose lanes all share one such source, and isolated
+    vertices keep rows equal to the oracle's under every policy."""
+    log(f"case: directed layered graph n=3000 with in-degree-0 sources, lane_words={words}")
+    rng = np.random.default_rng(21)
+    n = 3000
+    roots = np.arange(0, 200, dtype=np.int32)          # in-degree 0, out-edges only
+    mid = np.arange(200, 2600, dtype=np.int32)         # reached from roots and each other
+    u = np.concatenate([rng.choice(roots, 4000), rng.choice(mid, 9000)]).astype(np.int32)
+    v = np.concatenate([rng.choice(mid, 4000), rng.choice(mid, 9000)]).astype(np.int32)
+    h = dawn.HostGraph.from_csr(*_directed_csr(n, u, v), directed=True)  # 2600.. isolated
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=words)
+    lanes = 64 * words
+    batches = [
+        rng.choice(roots, lanes).astype(np.int32),
+        np.full(lanes, 7, np.int32),
+        np.concatenate([rng.choice(roots, lanes // 2), rng.choice(mid, lanes // 2)]).astype(np.int32),
+    ]
+    for srcs in batches:
+        odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+        for p in POLICIES:
+            dist, sums = g.mssp(srcs, policy=p)
+            assert np.array_equal(dist, odist), p
+            assert np.array_equal(sums, osums), p
