@@ -344,6 +344,20 @@ def main():
         log(f"{pol}: {results[pol]}")
     assert np.array_equal(sums_by_policy["learned"], sums_by_policy["fixed"])
 
+    # Closeness of the same 4096 sources: only the int64 sums are needed, so
+    # the engine writes no distance rows (not the headline workload).
+    g.closeness(src, policy="learned")
+    t0 = time.time()
+    cc_dev = []
+    for _ in range(args.steps):
+        _, cc_sums, _, st = g.closeness(src, policy="learned", stats=True)
+        cc_dev.append(st["device_ms"])
+    t = (time.time() - t0) / args.steps
+    assert np.array_equal(cc_sums, sums_by_policy["learned"])
+    results["closeness"] = {"ms_per_pass": 1e3 * t, "device_ms_per_pass": float(np.mean(cc_dev)),
+                            "sources_per_s": args.sources / t, "sums_equal_mssp": True}
+    log(f"closeness (sums only): {results['closeness']}")
+
     # Per-step oracle-best schedule on the first 4 batches: every mode timed
     # from identical state (dawn_sched_trace), then each policy's choices.
     sub = src[:64 * W]  # one batch at the timed run's width
@@ -468,6 +482,7 @@ def main():
         "sources_per_s": results["learned"]["sources_per_s"],
         "fixed": results["fixed"],
         "learned": results["learned"],
+        "closeness": results["closeness"],
         "schedule": sched,
         "roofline": roofline,
         "batch_roofline_frac": batch_frac,

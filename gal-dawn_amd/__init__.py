@@ -477,7 +477,10 @@ class Graph:
                "dawn_apsp_block")
         return dist, sums
 
-    def closeness(self, sources: Sequence[int], policy="fixed"):
+    def closeness(self, sources: Sequence[int], policy="fixed", stats: bool = False):
+        """Closeness (n-1)/sum per source, the exact int64 sums, and the
+        reference's integer-division value; unweighted graphs compute only
+        the sums (no distance rows)."""
         src = _i32(sources)
         S = len(src)
         cc = np.zeros(S, np.float64)
@@ -487,7 +490,7 @@ class Graph:
         _check(lib().dawn_closeness(self._h, _ptr(src, ctypes.c_int32), S, _ptr(cc, ctypes.c_double),
                                     _ptr(sums, ctypes.c_int64), _ptr(ref, ctypes.c_float),
                                     _policy(policy), ctypes.byref(st)), "dawn_closeness")
-        return cc, sums, ref
+        return (cc, sums, ref, st.as_dict()) if stats else (cc, sums, ref)
 
     PROF_NAMES = ("fill", "seed", "decide", "expand_push", "expand_pull", "expand_bitmap",
                   "finalize", "final")

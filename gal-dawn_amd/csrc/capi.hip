@@ -16,8 +16,10 @@ namespace dawn {
 int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out, int32_t policy,
                       dawn_stats* stats, int32_t force_mode, float* trace_feat,
                       int32_t* trace_mode, int32_t trace_max, int32_t* trace_rows);
+// rows: with dist null, still compute every distance row (into a scratch
+// buffer) rather than only the per-source sums.
 int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats);
+              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows);
 int run_wmsbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
                dawn_stats* stats);
 int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
@@ -96,7 +98,7 @@ int dawn_mssp(const dawn_graph* gc, const int32_t* sources, int32_t count, int32
   if (!g || count < 0 || (count > 0 && !sources)) return DAWN_ERR_INVALID_ARG;
   if (count == 0) return DAWN_OK;
   if (g->weighted) return run_wmsbfs(g, sources, count, dist, sums, stats);
-  return run_msbfs(g, sources, count, dist, sums, nullptr, nullptr, policy, stats);
+  return run_msbfs(g, sources, count, dist, sums, nullptr, nullptr, policy, stats, true);
 }
 
 int dawn_apsp_block(const dawn_graph* gc, int32_t first, int32_t count, int32_t* dist,
@@ -114,7 +116,11 @@ int dawn_closeness(const dawn_graph* gc, const int32_t* sources, int32_t count,
   dawn_graph* g = const_cast<dawn_graph*>(gc);
   if (!g || count < 0 || (count > 0 && !sources)) return DAWN_ERR_INVALID_ARG;
   std::vector<int64_t> s(count);
-  DAWN_TRY(dawn_mssp(g, sources, count, nullptr, s.data(), policy, stats));
+  if (count > 0) {
+    // Sums only: the unweighted engine skips the distance rows.
+    if (g->weighted) DAWN_TRY(run_wmsbfs(g, sources, count, nullptr, s.data(), stats));
+    else DAWN_TRY(run_msbfs(g, sources, count, nullptr, s.data(), nullptr, nullptr, policy, stats, false));
+  }
   for (int32_t i = 0; i < count; i++) {
     // (n-1) / sum, sum exact in int64, one IEEE double division
     // (src/algorithm/cpu/cc.cpp:77-78 divides by an int32 sum instead).

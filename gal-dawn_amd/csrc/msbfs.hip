@@ -1656,10 +1656,14 @@ void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t 
 template <int W>
 int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t* sources,
                    int32_t S, int32_t first, int32_t step, int32_t* dist, int64_t* sums,
-                   int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+                   int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats,
+                   bool rows) {
   constexpr int kB = 64 * W;
   const bool dist_dev = dist && is_device_pointer(dist);
-  if (!dist_dev) DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
+  // Rows of a host output, or (rows) of a call without output, go through
+  // the slot's row buffer; without either the distance pass only sums.
+  const bool buf_rows = !dist_dev && (dist || rows);
+  if (buf_rows) DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
 
   MsArgs a = make_args(g, ws);
   SchedParams p = make_params(g, policy);
@@ -1704,7 +1708,7 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
   float total_ms = 0.f;
   for (int32_t b0 = first * kB; b0 < S; b0 += step * kB) {
     const int32_t lanes = std::min<int32_t>(kB, S - b0);
-    int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : ws->distbuf.p;
+    int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : buf_rows ? ws->distbuf.p : nullptr;
     hipGraphExec_t exec;
     DAWN_TRY(get_graph(want_edges, &exec));
     stage_batch(ws, sources + b0, lanes, g->view);
@@ -1791,13 +1795,14 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
 // stats->device_ms is the wall time of the whole set of batches.
 template <int W>
 int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-                int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+                int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows) {
   constexpr int kB = 64 * W;
   DAWN_TRY(ensure_ms(g, g->ms, W));
   DAWN_TRY(ms_view_ensure(g));
   // Row buffers of batches whose rows do not go to a device buffer.
   const bool dist_dev = dist && is_device_pointer(dist);
-  if (!dist_dev) DAWN_TRY(g->ms->distbuf.alloc((size_t)kB * g->n));
+  const bool buf_rows = !dist_dev && (dist || rows);
+  if (buf_rows) DAWN_TRY(g->ms->distbuf.alloc((size_t)kB * g->n));
   bool two = S > kB;
   if (two) {
     if (!g->stream2 && hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking) != hipSuccess) {
@@ -1806,7 +1811,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       two = false;
     }
     if (two && (ensure_ms(g, g->ms2, W) != DAWN_OK ||
-                (!dist_dev && g->ms2->distbuf.alloc((size_t)kB * g->n) != DAWN_OK))) {
+                (buf_rows && g->ms2->distbuf.alloc((size_t)kB * g->n) != DAWN_OK))) {
       ms_workspace_free(g->ms2);  // not enough memory for a second batch in flight
       g->ms2 = nullptr;
       (void)hipGetLastError();
@@ -1815,7 +1820,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
   }
   if (!two)
     return run_msbfs_slot<W>(g, g->ms, g->stream, sources, S, 0, 1, dist, sums, reached, edges,
-                             policy, stats);
+                             policy, stats, rows);
   dawn_stats st1{};
   int rc1 = DAWN_OK;
   const auto t0 = std::chrono::steady_clock::now();
@@ -1825,11 +1830,11 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       return;
     }
     rc1 = run_msbfs_slot<W>(g, g->ms2, g->stream2, sources, S, 1, 2, dist, sums, reached, edges,
-                            policy, stats ? &st1 : nullptr);
+                            policy, stats ? &st1 : nullptr, rows);
   });
   dawn_stats st0{};
   const int rc0 = run_msbfs_slot<W>(g, g->ms, g->stream, sources, S, 0, 2, dist, sums, reached,
-                                    edges, policy, stats ? &st0 : nullptr);
+                                    edges, policy, stats ? &st0 : nullptr, rows);
   second.join();
   const double wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2010,7 +2015,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
 }  // namespace
 
 int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats) {
+              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows) {
   if (!g || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> guard(g->lock);
   DAWN_HIP(hipSetDevice(g->device));
@@ -2019,11 +2024,11 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, i
   for (int w = words_for(g, S);; w >>= 1) {
     int rc;
     switch (w) {
-      case 1: rc = run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
-      case 2: rc = run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
-      case 4: rc = run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
-      case 8: rc = run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
-      default: rc = run_msbfs_w<16>(g, sources, S, dist, sums, reached, edges, policy, stats); break;
+      case 1: rc = run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
+      case 2: rc = run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
+      case 4: rc = run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
+      case 8: rc = run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
+      default: rc = run_msbfs_w<16>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
     }
     if (rc != DAWN_ERR_OUT_OF_MEMORY || w == 1) return rc;
     ms_workspace_free(g->ms);

@@ -27,7 +27,7 @@
 namespace dawn {
 
 int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats);
+              int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows);
 int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out, int32_t policy,
                       dawn_stats* stats, int32_t force_mode, float* trace_feat,
                       int32_t* trace_mode, int32_t trace_max, int32_t* trace_rows);
@@ -57,7 +57,7 @@ int sample_ms(dawn_graph* g, const std::vector<int32_t>& src, double* ms) {
                                    0, nullptr));
     } else {
       DAWN_TRY(run_msbfs(g, src.data(), (int32_t)src.size(), nullptr, nullptr, nullptr, nullptr,
-                         DAWN_POLICY_FIXED, &st));
+                         DAWN_POLICY_FIXED, &st, true));
     }
     if (rep > 0) best = std::min(best, st.device_ms);
   }

@@ -157,7 +157,9 @@ int dawn_sssp(const dawn_graph* g, int32_t source, int32_t* dist,
  * bit-packed batch; weighted graphs run the min-plus engine per source.
  * sources are taken modulo n (reference: msource[i] % rows, mssp.cpp:15);
  * isolated sources are skipped (dist row left INF except dist[src]=0, sum 0).
- * dist: S*n int32 row-major (host or device), or NULL to keep only sums.
+ * dist: S*n int32 row-major (host or device), or NULL to keep only sums
+ * (the rows are still computed, each batch's into a device scratch buffer,
+ * as the reference computes every source's distance array and discards it).
  * sums: S int64 sums of distances over reached vertices, or NULL.
  * Reference: DAWN::MSSP_GPU::run / run_Weighted (src/algorithm/gpu/mssp.cu:9-102),
  * DAWN::MSSP_CPU::run (src/algorithm/cpu/mssp.cpp:9-32). */
@@ -175,6 +177,8 @@ int dawn_apsp_block(const dawn_graph* g, int32_t first, int32_t count,
  * division.  closeness/sums: count entries (host).  ref_int: if non-NULL,
  * receives the reference's unweighted value (row-1)/int32-sum with integer
  * division (src/algorithm/cpu/cc.cpp:77-78) as float.
+ * Unweighted graphs: the batched engine computes only the sums (no
+ * distance rows are written).
  * Reference: DAWN::CC_GPU::run (src/algorithm/gpu/cc.cu:9-103). */
 int dawn_closeness(const dawn_graph* g, const int32_t* sources, int32_t count,
                    double* closeness, int64_t* sums, float* ref_int,

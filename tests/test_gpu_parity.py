@@ -431,3 +431,29 @@ def test_zero_in_degree_sources_exact(orc, words):
             dist, sums = g.mssp(srcs, policy=p)
             assert np.array_equal(dist, odist), p
             assert np.array_equal(sums, osums), p
+
+
+def test_closeness_sums_only_wide_batches(orc):
+    """Closeness computes only the sums (no distance rows): two 1024-source
+    batches in flight, then an MSSP with rows on the same graph, then a
+    weighted graph's closeness; all equal the oracle."""
+    log("case: RMAT scale 15 ef 8 seed 23, 1100 sources, closeness then mssp, lane_words=16")
+    h = dawn.HostGraph.rmat(15, 8, 23)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=16)
+    srcs = dawn.gen_sources(h.n, 1100, 23)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    cc, sums, ref_int, st = g.closeness(srcs, policy="learned", stats=True)
+    assert np.array_equal(sums, osums)
+    assert st["batches"] == 2
+    for i in (0, 1, 555, 1099):
+        c, s, r = orc.closeness(odist[i])
+        assert cc[i] == c and ref_int[i] == r
+    dist, sums2 = g.mssp(srcs)
+    assert np.array_equal(dist, odist) and np.array_equal(sums2, osums)
+    hw = dawn.HostGraph.er(2000, 5.0, 24, weighted=True)
+    gw = dawn.Graph(hw)
+    ws = np.array([0, 7, 1999], np.int32)
+    wd, wsum, _, _ = orc.mssp(hw.row_ptr, hw.col, hw.weight, ws)
+    _, wsums, _ = gw.closeness(ws)
+    assert np.array_equal(wsums, wsum)
