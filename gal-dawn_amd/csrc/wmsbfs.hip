@@ -20,7 +20,8 @@
 //   q      int32[2][n]   vertices with any improved lane (step parity)
 // Step (push, chaotic like GOVMP): a frontier vertex u of out-degree < 16
 // gets one lane, which relaxes dist[v][b] = min(dist[v][b], dist[u][b] + w)
-// for each lane b active at u and each edge in turn; a heavier u gets one
+// for each lane b active at u and each edge (edge by edge, all lanes at
+// once from whole-record loads when L <= 16); a heavier u gets one
 // wave, split into 64/L groups of L lanes, group g taking edges g, g+64/L,
 // ... and lane b of a group relaxing source lane b.  Improved lanes are
 // OR-ed into act_next[v]; the first improvement appends v to the next
@@ -114,11 +115,73 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
   const int b = lane % L;
   const int grp = lane / L;
   const uint64_t gmask = L == 64 ? ~0ull : ((1ull << L) - 1);
-  // Light vertices (out-degree < kWLight): one lane per queued vertex, which
-  // walks its (active lane, edge) relaxations in turn -- on sparse graphs a
-  // vertex is usually active in few lanes, so a wave per vertex would leave
-  // most lanes idle.  The wave loops until its busiest lane is done, so the
-  // appends stay wave-aggregated.
+  // Light vertices (out-degree < kWLight): one lane per queued vertex -- on
+  // sparse graphs a vertex is usually active in few lanes, so a wave per
+  // vertex would leave most lanes idle.  L <= 16: the lane holds u's
+  // distances of all L sources (16-byte loads) and takes its edges in turn,
+  // loading each neighbour's record whole and issuing atomics only for the
+  // lanes that improve, one act_next OR per edge.  Wider batches walk the
+  // (active lane, edge) pairs one at a time.  The wave loops until its
+  // busiest lane is done, so the appends stay wave-aggregated.
+  if constexpr (L <= 16) {
+    for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < qn;
+         base += (int64_t)gridDim.x * kBlock) {  // wave-uniform
+      const int64_t i = base + lane;
+      int32_t u = 0;
+      uint64_t m = 0;
+      int beg = 0, deg = 0;
+      if (i < qn) {
+        u = q_in[i];
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+        if (deg < kWLight) m = act_in[u];
+      }
+      const bool light = i < qn && deg < kWLight;
+      const int most = wave_max_i32(m ? deg : 0);
+      int32_t du[L];
+      if (m) {
+        const int4* up = reinterpret_cast<const int4*>(a.dist + (int64_t)u * L);
+#pragma unroll
+        for (int t = 0; t < L / 4; t++) {
+          const int4 x = up[t];
+          du[4 * t] = x.x;
+          du[4 * t + 1] = x.y;
+          du[4 * t + 2] = x.z;
+          du[4 * t + 3] = x.w;
+        }
+      }
+      for (int e = 0; e < most; e++) {
+        bool append = false;
+        int32_t v = -1;
+        if (m && e < deg) {
+          v = a.col[beg + e];
+          const int32_t wt = a.w[beg + e];
+          int32_t* vp = a.dist + (int64_t)v * L;
+          int32_t dv[L];
+#pragma unroll
+          for (int t = 0; t < L / 4; t++) {
+            const int4 x = reinterpret_cast<const int4*>(vp)[t];
+            dv[4 * t] = x.x;
+            dv[4 * t + 1] = x.y;
+            dv[4 * t + 2] = x.z;
+            dv[4 * t + 3] = x.w;
+          }
+          uint64_t imp = 0;
+#pragma unroll
+          for (int bl = 0; bl < L; bl++) {
+            if (((m >> bl) & 1ull) && du[bl] != kInf) {
+              const int32_t nd = du[bl] + wt;
+              if (nd < dv[bl] && nd < atomicMin(vp + bl, nd)) imp |= 1ull << bl;
+            }
+          }
+          if (imp) append = atomicOr((unsigned long long*)&act_out[v], (unsigned long long)imp) == 0ull;
+        }
+        const int pos = wave_append(qlen_out, append);
+        if (append) q_out[pos] = v;
+      }
+      if (light) act_in[u] = 0ull;  // consumed
+    }
+  } else {
   for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < qn;
        base += (int64_t)gridDim.x * kBlock) {  // wave-uniform
     const int64_t i = base + lane;
@@ -160,6 +223,7 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
       if (append) q_out[pos] = v;
     }
     if (light) act_in[u] = 0ull;  // consumed
+  }
   }
   // Heavy vertices: one wave each, split into groups of L lanes.
   const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
