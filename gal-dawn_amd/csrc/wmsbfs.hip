@@ -20,8 +20,8 @@
 //   q      int32[2][n]   vertices with any improved lane (step parity)
 // Step (push, chaotic like GOVMP): a frontier vertex u of out-degree < 16
 // gets one lane, which relaxes dist[v][b] = min(dist[v][b], dist[u][b] + w)
-// for each lane b active at u and each edge (edge by edge, all lanes at
-// once from whole-record loads when L <= 16); a heavier u gets one
+// for each lane b active at u and each edge (edge by edge, 16 lanes at a
+// time from 16-byte record loads); a heavier u gets one
 // wave, split into 64/L groups of L lanes, group g taking edges g, g+64/L,
 // ... and lane b of a group relaxing source lane b.  Improved lanes are
 // OR-ed into act_next[v]; the first improvement appends v to the next
@@ -117,13 +117,27 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
   const uint64_t gmask = L == 64 ? ~0ull : ((1ull << L) - 1);
   // Light vertices (out-degree < kWLight): one lane per queued vertex -- on
   // sparse graphs a vertex is usually active in few lanes, so a wave per
-  // vertex would leave most lanes idle.  L <= 16: the lane holds u's
-  // distances of all L sources (16-byte loads) and takes its edges in turn,
-  // loading each neighbour's record whole and issuing atomics only for the
-  // lanes that improve, one act_next OR per edge.  Wider batches walk the
-  // (active lane, edge) pairs one at a time.  The wave loops until its
-  // busiest lane is done, so the appends stay wave-aggregated.
-  if constexpr (L <= 16) {
+  // vertex would leave most lanes idle.  The lane takes u's edges in turn;
+  // per edge it works through the record in chunks of CH = min(L, 16)
+  // source lanes that hold an active lane of u: 16-byte loads of u's and
+  // the neighbour's distances (u's held in registers when L <= 16), atomics
+  // only for the lanes that improve, and one act_next OR per edge.  The
+  // wave loops until its busiest lane is done, so the appends stay
+  // wave-aggregated.
+  {
+    constexpr int CH = L < 16 ? L : 16;
+    constexpr int NC = L / CH;
+    constexpr uint64_t kChunk = CH == 64 ? ~0ull : ((1ull << CH) - 1);
+    auto load_chunk = [](const int32_t* p, int32_t (&x)[CH]) {
+#pragma unroll
+      for (int t = 0; t < CH / 4; t++) {
+        const int4 y = reinterpret_cast<const int4*>(p)[t];
+        x[4 * t] = y.x;
+        x[4 * t + 1] = y.y;
+        x[4 * t + 2] = y.z;
+        x[4 * t + 3] = y.w;
+      }
+    };
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < qn;
          base += (int64_t)gridDim.x * kBlock) {  // wave-uniform
       const int64_t i = base + lane;
@@ -138,18 +152,9 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
       }
       const bool light = i < qn && deg < kWLight;
       const int most = wave_max_i32(m ? deg : 0);
-      int32_t du[L];
-      if (m) {
-        const int4* up = reinterpret_cast<const int4*>(a.dist + (int64_t)u * L);
-#pragma unroll
-        for (int t = 0; t < L / 4; t++) {
-          const int4 x = up[t];
-          du[4 * t] = x.x;
-          du[4 * t + 1] = x.y;
-          du[4 * t + 2] = x.z;
-          du[4 * t + 3] = x.w;
-        }
-      }
+      const int32_t* up = a.dist + (int64_t)u * L;
+      int32_t du[CH];
+      if (NC == 1 && m) load_chunk(up, du);
       for (int e = 0; e < most; e++) {
         bool append = false;
         int32_t v = -1;
@@ -157,21 +162,20 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
           v = a.col[beg + e];
           const int32_t wt = a.w[beg + e];
           int32_t* vp = a.dist + (int64_t)v * L;
-          int32_t dv[L];
-#pragma unroll
-          for (int t = 0; t < L / 4; t++) {
-            const int4 x = reinterpret_cast<const int4*>(vp)[t];
-            dv[4 * t] = x.x;
-            dv[4 * t + 1] = x.y;
-            dv[4 * t + 2] = x.z;
-            dv[4 * t + 3] = x.w;
-          }
           uint64_t imp = 0;
+#pragma unroll 1
+          for (int c = 0; c < NC; c++) {
+            const uint64_t mc = (m >> (c * CH)) & kChunk;
+            if (!mc) continue;
+            if (NC > 1) load_chunk(up + c * CH, du);
+            int32_t dv[CH];
+            load_chunk(vp + c * CH, dv);
 #pragma unroll
-          for (int bl = 0; bl < L; bl++) {
-            if (((m >> bl) & 1ull) && du[bl] != kInf) {
-              const int32_t nd = du[bl] + wt;
-              if (nd < dv[bl] && nd < atomicMin(vp + bl, nd)) imp |= 1ull << bl;
+            for (int bl = 0; bl < CH; bl++) {
+              if (((mc >> bl) & 1ull) && du[bl] != kInf) {
+                const int32_t nd = du[bl] + wt;
+                if (nd < dv[bl] && nd < atomicMin(vp + c * CH + bl, nd)) imp |= 1ull << (c * CH + bl);
+              }
             }
           }
           if (imp) append = atomicOr((unsigned long long*)&act_out[v], (unsigned long long)imp) == 0ull;
@@ -181,49 +185,6 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
       }
       if (light) act_in[u] = 0ull;  // consumed
     }
-  } else {
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < qn;
-       base += (int64_t)gridDim.x * kBlock) {  // wave-uniform
-    const int64_t i = base + lane;
-    int32_t u = 0;
-    uint64_t m = 0;
-    int beg = 0, deg = 0;
-    if (i < qn) {
-      u = q_in[i];
-      beg = a.row_ptr[u];
-      deg = a.row_ptr[u + 1] - beg;
-      if (deg < kWLight) m = act_in[u];
-    }
-    const bool light = i < qn && deg < kWLight;
-    const int work = __popcll(m) * deg;
-    const int most = wave_max_i32(work);
-    int e = 0, bl = m ? __ffsll((unsigned long long)m) - 1 : 0;
-    int32_t du = m ? a.dist[(int64_t)u * L + bl] : kInf;
-    for (int it = 0; it < most; it++) {
-      bool append = false;
-      int32_t v = -1;
-      if (it < work) {
-        v = a.col[beg + e];
-        if (du != kInf) {
-          const int32_t nd = du + a.w[beg + e];
-          int32_t* dv = a.dist + (int64_t)v * L + bl;
-          if (nd < *dv && nd < atomicMin(dv, nd))
-            append = atomicOr((unsigned long long*)&act_out[v], 1ull << bl) == 0ull;
-        }
-        if (++e == deg) {  // next active lane of u
-          e = 0;
-          m &= m - 1;
-          if (m) {
-            bl = __ffsll((unsigned long long)m) - 1;
-            du = a.dist[(int64_t)u * L + bl];
-          }
-        }
-      }
-      const int pos = wave_append(qlen_out, append);
-      if (append) q_out[pos] = v;
-    }
-    if (light) act_in[u] = 0ull;  // consumed
-  }
   }
   // Heavy vertices: one wave each, split into groups of L lanes.
   const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
