@@ -457,3 +457,20 @@ def test_closeness_sums_only_wide_batches(orc):
     wd, wsum, _, _ = orc.mssp(hw.row_ptr, hw.col, hw.weight, ws)
     _, wsums, _ = gw.closeness(ws)
     assert np.array_equal(wsums, wsum)
+
+
+def test_all_vertex_closeness_exact(orc):
+    """Every vertex a source (isolated ones included), sums only: closeness,
+    int64 sums and the reference's integer-division value equal the
+    oracle's for every vertex."""
+    log("case: ER n=3000 deg 3 seed 25, closeness of all 3000 vertices, lane_words=16")
+    h = dawn.HostGraph.er(3000, 3.0, 25)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=16)
+    everyone = np.arange(h.n, dtype=np.int32)
+    cc, sums, ref_int = g.closeness(everyone, policy="learned")
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, everyone)
+    assert np.array_equal(sums, osums)
+    assert (h.degrees() == 0).any()
+    for v in range(h.n):
+        assert (cc[v], sums[v], ref_int[v]) == orc.closeness(odist[v]), v
