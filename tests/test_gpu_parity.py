@@ -267,6 +267,8 @@ def test_long_traversals_flush_windows(orc, words, count):
         dist, sums = g.mssp(srcs, policy=p)
         assert np.array_equal(dist, odist), p
         assert np.array_equal(sums, osums), p
+    _, csums, _ = g.closeness(srcs, policy="learned")  # sums only, every window
+    assert np.array_equal(csums, osums)
 
 
 @pytest.mark.parametrize("n", [10, 100])
