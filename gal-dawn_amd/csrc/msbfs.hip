@@ -915,30 +915,39 @@ __global__ void __launch_bounds__(kBlock) ms_pull_kernel(MsArgs a) {
       // in-neighbours found in the frontier are listed in LDS, and lane
       // pairs gather the listed records (two halves of one line each).
       // Sparse frontiers cost one probe per edge, not one per lane pair.
+      // The column reads and bitmap probes of R rounds are issued before
+      // any is examined, so a wave waits once per R*64 edges, not per round.
+      constexpr int R = 8;
       int* hits = s_hits + (threadIdx.x / kWave) * kWave;
-      for (int k0 = a.hc_beg[c]; k0 < he; k0 += kWave) {  // wave-uniform
-        const int k = k0 + lane;
-        int32_t u = 0;
-        bool hit = false;
-        if (k < he) {
-          u = a.in_col[k];
-          hit = (fb[u >> 5] >> (u & 31)) & 1u;
-        }
-        const uint64_t m = __ballot(hit);
-        if (!m) continue;
-        if (hit) hits[__popcll(m & lanes_below_mask())] = u;
-        wave_lds_sync();
-        const int nh = __popcll(m);
-        for (int b = 0; b < nh; b += kWave / TPV) {
-          const int i = b + lane / TPV;
-          if (i < nh) {
-            uint64_t f[HW];
-            load_words<HW>(prev + (int64_t)hits[i] * W + h * HW, f);
+      for (int k0 = a.hc_beg[c]; k0 < he; k0 += R * kWave) {  // wave-uniform
+        int32_t u[R];
 #pragma unroll
-            for (int t = 0; t < HW; t++) part[t] |= f[t];
-          }
+        for (int r = 0; r < R; r++) {
+          const int k = k0 + r * kWave + lane;
+          u[r] = k < he ? a.in_col[k] : -1;
         }
-        wave_lds_sync();  // the list is rewritten by the next round
+        uint32_t fw[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) fw[r] = u[r] >= 0 ? fb[u[r] >> 5] : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const bool hit = u[r] >= 0 && ((fw[r] >> (u[r] & 31)) & 1u);
+          const uint64_t m = __ballot(hit);
+          if (!m) continue;
+          if (hit) hits[__popcll(m & lanes_below_mask())] = u[r];
+          wave_lds_sync();
+          const int nh = __popcll(m);
+          for (int b = 0; b < nh; b += kWave / TPV) {
+            const int i = b + lane / TPV;
+            if (i < nh) {
+              uint64_t f[HW];
+              load_words<HW>(prev + (int64_t)hits[i] * W + h * HW, f);
+#pragma unroll
+              for (int t = 0; t < HW; t++) part[t] |= f[t];
+            }
+          }
+          wave_lds_sync();  // the list is rewritten by the next round
+        }
       }
     }
 #pragma unroll
