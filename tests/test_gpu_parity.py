@@ -476,3 +476,20 @@ def test_all_vertex_closeness_exact(orc):
     assert (h.degrees() == 0).any()
     for v in range(h.n):
         assert (cc[v], sums[v], ref_int[v]) == orc.closeness(odist[v]), v
+
+
+@pytest.mark.parametrize("heavy,chunk", [(48, 1024), (48, 1000), (32, 512)])
+def test_heavy_chunks_multi_round_exact(orc, heavy, chunk):
+    """Heavy in-edge chunks longer and shorter than one round of prefetched
+    probes (512 edges at 1024 sources per batch), ragged chunk tails and
+    vertices split over several chunks: rows and sums equal the oracle's."""
+    log(f"case: RMAT scale 16 ef 16 seed 31, 1024 sources, heavy_degree={heavy}, pull_chunk={chunk}")
+    h = dawn.HostGraph.rmat(16, 16, 31)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=16, heavy_degree=heavy, pull_chunk=chunk)
+    srcs = dawn.gen_sources(h.n, 1024, 31)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    for p in ("pull", "learned"):
+        dist, sums = g.mssp(srcs, policy=p)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
