@@ -995,6 +995,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
     unsigned dnib = kAll;
     if (v0 < a.npad) dnib = (a.done[v0 >> 5] >> (v0 & 31)) & kAll;
     const bool work = v0 < a.npad && dnib != kAll;  // same for the threads of a vertex
+    const unsigned done_in = dnib;  // (W < 4) siblings done on entry are not counted again
     const int64_t w0 = v0 * W + WPT * sub;
     ulonglong2* np2 = reinterpret_cast<ulonglong2*>(a.next + w0);
     ulonglong2* sp2 = reinterpret_cast<ulonglong2*>(a.seen + w0);
@@ -1070,9 +1071,11 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
             acc.scout += (unsigned)(d > 0 ? d : 1);
             acc.lanes += pc[j];
           }
-          if (unv[j]) {  // still misses a live lane
-            acc.uv += 1;
-            acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+          if (unv[j]) {  // still misses a live lane (or started done: in-degree 0)
+            if (!((done_in >> j) & 1u)) {
+              acc.uv += 1;
+              acc.ue += (unsigned)(a.in_row_ptr[v + 1] - a.in_row_ptr[v]);
+            }
           } else {
             dnib |= 1u << j;
           }
