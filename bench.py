@@ -151,10 +151,13 @@ def run_suite(dawn, orc, steps):
     dist, _ = g.mssp(src)
     ok = bool(np.array_equal(dist, orc.mssp(h.row_ptr, h.col, None, src)[0]))
     t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src, threads)
+    # The other CPU mode: single-source dense OpenMP BOVM (the oracle's port
+    # of BFS_CPU::BFS, src/algorithm/cpu/bfs.cpp:23-77), best of 3.
+    t_dense = min(orc.time_bfs_dense(h.row_ptr, h.col, int(src[0]), threads) for _ in range(3))
     print(json.dumps({"config": "er65k: ER n=65,536 deg 8 seed 1, source 0 + 64", "bfs_ms": bfs_ms,
                       "mssp_ms": ms_ms, "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
                       "cpu_reference_sources_per_s": len(src) / t_cpu, "cpu_threads": threads,
-                      "rows_bit_exact": ok}), flush=True)
+                      "cpu_dense_bfs_ms": 1e3 * t_dense, "rows_bit_exact": ok}), flush=True)
     g.close()
 
     # grid2048: weighted single-source SSSP (int32 min-plus) and 16-source MSSP.
