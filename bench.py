@@ -1,12 +1,15 @@
 """Benchmark: unweighted MSSP on RMAT scale 24, edge factor 16 (symmetrized,
 deduplicated, seed 99), 4096 uniform sources (seed 99), on one MI355X.
 
-A step = one MSSP pass over all sources (64 bit-packed batches of 64).
-Reported value: GTEPS of the learned schedule, whole job, inputs resident
-in HBM.  Also: sources/s, the fixed alpha/beta schedule's numbers, the
-per-step oracle-best schedule and regret, a bit-exact spot check against
-the CPU oracle, the dominant kernel's roofline, and the reference's own CPU
-MSSP (oracle/_ref) timed on a bounded sample of the same workload.
+A step = one MSSP pass over all sources: bit-packed batches of 64*W sources
+(W = the lane_words knob, 16 from the committed tune cache: 4 batches of
+1024), two batches in flight, every batch's int32 distance rows written to
+HBM.  Reported value: GTEPS of the learned schedule, whole job, inputs
+resident in HBM.  Also: sources/s, the fixed alpha/beta schedule's numbers,
+the closeness of the same sources (sums only), the per-step oracle-best
+schedule and regret, a bit-exact spot check against the CPU oracle, the
+dominant kernel's roofline, and the reference's own CPU MSSP (oracle/_ref)
+timed on a bounded sample of the same workload.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24]
 """
