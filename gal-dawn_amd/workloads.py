@@ -4,7 +4,7 @@ er65k         ER n=65,536, avg degree 8, seed 1; source 0 + 64 uniform sources
 grid2048      2048x2048 4-neighbour grid + 1% random long-range edges, int32
               weights uniform in [1,1000], seed 2; 16 uniform sources
 rmat20        RMAT(.57,.19,.19,.05) scale 20, ef 16, symmetrized, dedup,
-              seed 3; 4096 uniform sources (64 batches of 64 lanes)
+              seed 3; 4096 uniform sources (bit-packed batches of 64*W lanes)
 uneven512     512 graphs, seed 4: ER (avg degree 4-32), RMAT (scale 10-17),
               grids with 1% long-range edges; vertex counts log-uniform in
               [1k, 130k]; 8 sources each; unweighted or int32-weighted

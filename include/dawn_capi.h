@@ -153,8 +153,10 @@ int dawn_bfs(const dawn_graph* g, int32_t source, int32_t* dist,
 int dawn_sssp(const dawn_graph* g, int32_t source, int32_t* dist,
               int32_t policy, dawn_stats* stats);
 
-/* Multi-source shortest paths.  Unweighted graphs run 64 sources per
- * bit-packed batch; weighted graphs run the min-plus engine per source.
+/* Multi-source shortest paths.  Unweighted graphs run 64*W sources per
+ * bit-packed batch (W = the lane-words knob, up to 16: 1024 sources);
+ * weighted graphs run the batched min-plus engine, up to 64 sources per
+ * batch.
  * sources are taken modulo n (reference: msource[i] % rows, mssp.cpp:15);
  * isolated sources are skipped (dist row left INF except dist[src]=0, sum 0).
  * dist: S*n int32 row-major (host or device), or NULL to keep only sums
