@@ -135,12 +135,35 @@ def summarize_rocprof(out_dir, profiles_dir):
     return summary, "\n".join(lines)
 
 
+def host_cpu():
+    """Threads for the CPU baseline and what the host reports: the OpenMP
+    thread count the process is given (OMP_NUM_THREADS, else every logical
+    core it may run on), nproc, the CPU model."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else usable
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": nproc, "affinity_cpus": usable,
+            "omp_num_threads_env": env or None, "cpu_model": model}
+
+
 def run_suite(dawn, orc, steps):
     """The other configurations, one JSON line each (GPU device time, best of
     `steps` runs after one warm-up; rows checked against the oracle on a
     sample; the reference's CPU MSSP timed beside where it is correct)."""
     wl = importlib.import_module("gal_dawn_amd.workloads")
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cpu()["threads"]
 
     def best(fn):
         fn()
@@ -440,9 +463,6 @@ def main():
     b_batch = batch_bytes(n, m, L, W)
     lt = results["learned"]["device_ms_per_step"] / 1e3
     batch_frac = (b_batch * (args.sources / (64 * W))) / lt / 1e9 / HBM_PEAK_GBS
-    # The 64-source figure of the same run: what the work would move in
-    # 64-lane batches (the bit-packing width the metric is defined on).
-    batch_frac_64 = (batch_bytes(n, m, L) * (args.sources / 64)) / lt / 1e9 / HBM_PEAK_GBS
     log(f"kernels: {kern}; levels/batch {L:.2f}; dominant {dom}")
 
     # Bit-exact spot check against the CPU oracle on sampled sources.
@@ -451,7 +471,7 @@ def main():
     chk = src[np.linspace(0, args.sources - 1, max(args.check, 1)).astype(int)]
     if args.check > 0:
         dist, sums = g.mssp(chk, policy="learned")
-        odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, chk, threads=min(16, os.cpu_count() or 1))
+        odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, chk, threads=host_cpu()["threads"])
         parity = bool(np.array_equal(dist, odist) and np.array_equal(sums, osums))
         pos = [int(np.flatnonzero(src == c)[0]) for c in chk]
         parity_sums = bool(np.array_equal(sums_by_policy["learned"][pos], osums))
@@ -459,7 +479,8 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        threads = host["threads"]
         sample = args.cpu_sample or 2 * threads
         csrc = src[:sample]
         if orc.ref_available():
@@ -473,8 +494,15 @@ def main():
             "unit": "sources/s",
             "cores": threads,
             "kind": kind,
-            "sample": f"{sample} of the {args.sources} sources, DAWN::MSSP_CPU::run (one source per "
-                      f"OpenMP thread, sequential sparse BFS), wall {t:.2f}s",
+            "sample": f"the first {sample} of the {args.sources} sources, DAWN::MSSP_CPU::run (one "
+                      f"source per OpenMP thread, sequential sparse BFS), {threads} threads",
+            "sample_sources": int(sample),
+            "sample_wall_s": t,
+            "extrapolated_all_sources_s": t * args.sources / sample,
+            "nproc": host["nproc"],
+            "affinity_cpus": host["affinity_cpus"],
+            "omp_num_threads_env": host["omp_num_threads_env"],
+            "cpu_model": host["cpu_model"],
             # GTEPS-equivalent: the average reached edges per source of this workload.
             "gteps": results["learned"]["edges_reached"] / args.sources * sample / t / 1e9,
         }
@@ -506,7 +534,6 @@ def main():
         "schedule": sched,
         "roofline": roofline,
         "batch_roofline_frac": batch_frac,
-        "batch_roofline_frac_64lane_bytes": batch_frac_64,
         "lane_words": W,
         "kernels": kern,
         "levels_per_batch": L,

@@ -162,6 +162,14 @@ namespace dawn {
 
 // ---- wave64 device helpers -------------------------------------------------
 
+// Min-plus candidate du + w, summed in 64 bits: kInf when u is unreached or
+// the path weight reaches INT32_MAX (such a path is not representable in
+// int32 and is ignored, as the CPU restatement skips sums >= INT32_MAX).
+__device__ __forceinline__ int32_t minplus(int32_t du, int32_t w) {
+  const long long nd = (long long)du + (long long)w;
+  return nd >= (long long)kInf ? kInf : (int32_t)nd;
+}
+
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 __device__ __forceinline__ uint64_t lanes_below_mask() {

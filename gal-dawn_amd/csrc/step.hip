@@ -61,8 +61,7 @@ __device__ __forceinline__ bool mark(uint32_t* beta, int32_t v) {
 __device__ __forceinline__ int32_t through(const StepArgs& a, int32_t u, const int32_t* w,
                                            int64_t e) {
   if (!a.weight) return a.level;
-  const int32_t du = a.dist[u];
-  return du == kInf ? kInf : du + w[e];
+  return minplus(a.dist[u], w[e]);
 }
 
 // Relaxes v with candidate d (atomicMin); counts v on its first marking.

@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
 #pragma unroll
             for (int bl = 0; bl < CH; bl++) {
               if (((mc >> bl) & 1ull) && du[bl] != kInf) {
-                const int32_t nd = du[bl] + wt;
+                const int32_t nd = minplus(du[bl], wt);
                 if (nd < dv[bl] && nd < atomicMin(vp + c * CH + bl, nd)) imp |= 1ull << (c * CH + bl);
               }
             }
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
       if (e < end) {
         v = a.col[e];
         if (active && du != kInf) {
-          const int32_t nd = du + a.w[e];
+          const int32_t nd = minplus(du, a.w[e]);
           int32_t* dv = a.dist + (int64_t)v * L + b;
           if (nd < *dv) improved = nd < atomicMin(dv, nd);
         }

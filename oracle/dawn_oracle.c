@@ -590,3 +590,47 @@ double oracle_mlp_loss_grad(const double* p, const double* x, const double* y,
   }
   return loss;
 }
+
+/* ---- shortest-path certificate (no reference counterpart) --------------- */
+
+/* The textbook optimality conditions of single-source shortest paths with
+ * positive weights (w = 1 when w is NULL).  For reached u (d[u] < INF) and
+ * every edge u->v: d[v] <= d[u] + w (a sum at or above INF bounds nothing,
+ * as the engine and the restatement above ignore such paths); every reached
+ * v != source has an in-edge u->v with d[u] + w == d[v]; d[source] == 0.
+ * Together they fix d: the parent edges give a path of weight d[v] (d drops
+ * by >= 1 per edge, so the chain ends at the source), and the edge
+ * inequality bounds d[v] by every path's weight; an unreached vertex with a
+ * reachable in-neighbour breaks the inequality.  O(m), OpenMP over rows. */
+int oracle_certify(int32_t n, const int32_t* row_ptr, const int32_t* col,
+                   const int32_t* w, int32_t source, const int32_t* d,
+                   int32_t threads) {
+  if (n <= 0 || source < 0 || source >= n) return -1;
+  if (d[source] != 0) return 0;
+  unsigned char* parent = (unsigned char*)calloc((size_t)n, 1);
+  if (!parent) return -5;
+  int nth = threads > 0 ? threads : omp_get_max_threads();
+  int ok = 1, bad_weight = 0;
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 4096) reduction(&& : ok) reduction(|| : bad_weight)
+  for (int32_t u = 0; u < n; u++) {
+    if (d[u] < 0) { ok = 0; continue; }
+    if (d[u] == ORACLE_INF) continue;
+    for (int32_t k = row_ptr[u]; k < row_ptr[u + 1]; k++) {
+      const int32_t wk = w ? w[k] : 1;
+      if (wk <= 0) { bad_weight = 1; continue; }
+      const int32_t v = col[k];
+      const int64_t through = (int64_t)d[u] + (int64_t)wk;
+      if (through >= ORACLE_INF) continue;
+      if ((int64_t)d[v] > through) ok = 0;
+      if ((int64_t)d[v] == through) {
+#pragma omp atomic write
+        parent[v] = 1;
+      }
+    }
+  }
+  if (bad_weight) { free(parent); return -1; }
+  for (int32_t v = 0; v < n && ok; v++)
+    if (v != source && d[v] != ORACLE_INF && !parent[v]) ok = 0;
+  free(parent);
+  return ok;
+}

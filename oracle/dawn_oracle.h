@@ -86,6 +86,15 @@ int oracle_dobfs_modes(int32_t n, const int32_t* row_ptr, const int32_t* col,
                        int32_t source, int32_t alpha, int32_t beta,
                        int32_t* modes, int32_t max);
 
+/* Shortest-path certificate (the optimality conditions; no reference
+ * counterpart): returns 1 iff d (engine encoding) is the distance array from
+ * source over the out-edges with positive weights w (NULL: unit weights),
+ * 0 if not, -1 for invalid arguments or a weight <= 0.  Paths of weight >=
+ * INT32_MAX count as absent.  O(m), OpenMP (threads <= 0: default). */
+int oracle_certify(int32_t n, const int32_t* row_ptr, const int32_t* col,
+                   const int32_t* w, int32_t source, const int32_t* d,
+                   int32_t threads);
+
 /* 8->32->32->3 ReLU MLP in double precision.  params layout:
  * W1[32][8] b1[32] W2[32][32] b2[32] W3[3][32] b3[3] (1443 values).
  * loss = mean over N*3 of (out - y)^2. */

@@ -51,6 +51,7 @@ def lib():
             ("oracle_closeness", c.c_double, [c.c_int32, _I32P, _I64P, _F32P]),
             ("oracle_dobfs_modes", c.c_int, [c.c_int32, _I32P, _I32P, c.c_int32, c.c_int32, c.c_int32,
                                              _I32P, c.c_int32]),
+            ("oracle_certify", c.c_int, [c.c_int32, _I32P, _I32P, _I32P, c.c_int32, _I32P, c.c_int32]),
             ("oracle_mlp_forward", None, [_F64P, _F64P, c.c_int32, _F64P]),
             ("oracle_mlp_loss_grad", c.c_double, [_F64P, _F64P, _F64P, c.c_int32, _F64P]),
         ]:
@@ -107,7 +108,9 @@ def bfs(row_ptr, col, source, dense=True, encoding="inf"):
     return dist
 
 
-def sssp(row_ptr, col, w, source, dense=True):
+def sssp(row_ptr, col, w, source, dense=True, overflow_ok=False):
+    """CPU DAWN min-plus SSSP.  A path whose weight reaches INT32_MAX is
+    skipped (status -8); overflow_ok accepts that and returns the row."""
     rp, cc, ww = _csr(row_ptr, col, w)
     n = len(rp) - 1
     dist = np.zeros(n, np.int32)
@@ -115,24 +118,24 @@ def sssp(row_ptr, col, w, source, dense=True):
     fn = lib().oracle_sssp_dense if dense else lib().oracle_sssp_sparse
     rc = fn(n, _p(rp, ctypes.c_int32), _p(cc, ctypes.c_int32), _p(ww, ctypes.c_int32), int(source),
             _p(dist, ctypes.c_int32), ctypes.byref(steps))
-    if rc != 0:
+    if rc != 0 and not (overflow_ok and rc == -8):
         raise ValueError(f"oracle sssp status {rc}")
     return dist
 
 
-def dijkstra(row_ptr, col, w, source):
+def dijkstra(row_ptr, col, w, source, overflow_ok=False):
     rp, cc, ww = _csr(row_ptr, col, w)
     n = len(rp) - 1
     dist = np.zeros(n, np.int32)
     rc = lib().oracle_dijkstra(n, _p(rp, ctypes.c_int32), _p(cc, ctypes.c_int32),
                                None if ww is None else _p(ww, ctypes.c_int32), int(source),
                                _p(dist, ctypes.c_int32))
-    if rc != 0:
+    if rc != 0 and not (overflow_ok and rc == -8):
         raise ValueError(f"oracle dijkstra status {rc}")
     return dist
 
 
-def mssp(row_ptr, col, w, sources, want_dist=True, threads=0):
+def mssp(row_ptr, col, w, sources, want_dist=True, threads=0, overflow_ok=False):
     rp, cc, ww = _csr(row_ptr, col, w)
     n = len(rp) - 1
     src = np.ascontiguousarray(sources, np.int32)
@@ -145,9 +148,24 @@ def mssp(row_ptr, col, w, sources, want_dist=True, threads=0):
                            None if ww is None else _p(ww, ctypes.c_int32), _p(src, ctypes.c_int32), S,
                            None if dist is None else _p(dist, ctypes.c_int32), _p(sums, ctypes.c_int64),
                            _p(skipped, ctypes.c_int32), threads, ctypes.byref(secs))
-    if rc != 0:
+    if rc != 0 and not (overflow_ok and rc == -8):
         raise ValueError(f"oracle mssp status {rc}")
     return dist, sums, skipped, secs.value
+
+
+def certify(row_ptr, col, w, source, dist, threads=0):
+    """True iff dist is the shortest-path distance array from source
+    (optimality conditions, O(m)); w None = unit weights (BFS hops)."""
+    rp, cc, ww = _csr(row_ptr, col, w)
+    d = np.ascontiguousarray(dist, np.int32)
+    n = len(rp) - 1
+    assert len(d) == n
+    rc = lib().oracle_certify(n, _p(rp, ctypes.c_int32), _p(cc, ctypes.c_int32),
+                              None if ww is None else _p(ww, ctypes.c_int32), int(source),
+                              _p(d, ctypes.c_int32), int(threads))
+    if rc < 0:
+        raise ValueError(f"oracle certify status {rc}")
+    return rc == 1
 
 
 def closeness(dist_row):

@@ -66,12 +66,58 @@ def test_sssp_gpu_main_output_file(mtx, orc):
         assert got[j] == pytest.approx(exp[j], rel=1e-6)
 
 
+def weighted_lines(path):
+    out = {}
+    for line in open(path).read().splitlines():
+        a, b, c = line.split()
+        out[int(b)] = float(c)
+    return out
+
+
 def test_mssp_gpu_main_source_list(mtx, orc, tmp_path):
+    """The mssp_gpu main writes the row of graph.source when it is in the
+    list (src/algorithm/gpu/bfs.cu:97-102); list entries are taken mod n."""
     _, p, _ = mtx
+    g = dawn.HostGraph.read_mtx(p)
     lst = tmp_path / "src.txt"
     lst.write_text("% sources\n" + "\n".join(str(x) for x in [3, 4099, 2147483647, 77]) + "\n")
-    text = run("mssp_gpu", "-i", p, "-l", str(lst), "-s", "3")
+    src = 4099 % g.n
+    assert g.degrees()[src] > 0
+    out = str(tmp_path / "mssp.txt")
+    text = run("mssp_gpu", "-i", p, "-l", str(lst), "-s", str(src), "-p", "true", "-o", out)
+    assert "Time:" in text and "Start print" in text
+    assert open(out).read() == expected_lines(src, orc.bfs(g.row_ptr, g.col, src))
+
+
+def test_mssp_gpu_main_weighted_row(mtx, orc, tmp_path):
+    _, _, pw = mtx
+    g = dawn.HostGraph.read_mtx(pw, weighted=True)
+    lst = tmp_path / "wsrc.txt"
+    lst.write_text("5\n17\n900\n")
+    out = str(tmp_path / "wmssp.txt")
+    run("mssp_gpu", "-i", pw, "-l", str(lst), "-s", "17", "-p", "true", "-o", out, "-w", "true")
+    dist = orc.dijkstra(g.row_ptr, g.col, g.weight, 17)
+    got = weighted_lines(out)
+    exp = {j: float(x) for j, x in enumerate(dist) if j != 17 and x != dawn.INF}
+    assert got == pytest.approx(exp, rel=1e-6)
+
+
+def test_apsp_gpu_main_rows(mtx, orc, tmp_path):
+    """The apsp_gpu main runs every vertex as a source and writes the row of
+    graph.source (src/algorithm/gpu/apsp.cu:9-60, bfs.cu:97-102)."""
+    _, p, pw = mtx
+    g = dawn.HostGraph.read_mtx(p)
+    src = int(np.argmax(g.degrees()))
+    out = str(tmp_path / "apsp.txt")
+    text = run("apsp_gpu", "-i", p, "-o", out, "-p", "true", "-s", str(src))
     assert "Time:" in text
+    assert open(out).read() == expected_lines(src, orc.bfs(g.row_ptr, g.col, src))
+    gw = dawn.HostGraph.read_mtx(pw, weighted=True)
+    outw = str(tmp_path / "wapsp.txt")
+    run("apsp_gpu", "-i", pw, "-o", outw, "-p", "true", "-s", "33", "-w", "true")
+    dist = orc.dijkstra(gw.row_ptr, gw.col, gw.weight, 33)
+    exp = {j: float(x) for j, x in enumerate(dist) if j != 33 and x != dawn.INF}
+    assert weighted_lines(outw) == pytest.approx(exp, rel=1e-6)
 
 
 def test_cc_gpu_main_value(mtx, orc):

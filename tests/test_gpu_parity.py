@@ -493,3 +493,53 @@ def test_heavy_chunks_multi_round_exact(orc, heavy, chunk):
         dist, sums = g.mssp(srcs, policy=p)
         assert np.array_equal(dist, odist), p
         assert np.array_equal(sums, osums), p
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_apsp_block_rows_equal_oracle(orc, weighted):
+    """dawn_apsp_block (APSP_GPU::run, src/algorithm/gpu/apsp.cu:9-111):
+    the rows of sources [first, first+count) equal the oracle's, for the
+    whole vertex range in one call and for blocks that start and end inside
+    a batch; isolated vertices get INF rows with 0 at the source."""
+    log(f"case: APSP blocks, weighted={weighted}")
+    if weighted:
+        h = dawn.HostGraph.grid(36, 0.02, 41, weighted=True)  # 1296 vertices
+    else:
+        h = dawn.HostGraph.rmat(11, 8, 41)  # 2048 vertices, isolated ones included
+    g = dawn.Graph(h)
+    everyone = np.arange(h.n, dtype=np.int32)
+    odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, h.weight, everyone)
+    dist, sums = g.apsp_block(0, h.n)
+    assert np.array_equal(dist, odist)
+    assert np.array_equal(sums, osums)
+    for first, count in ((5, 70), (1000, 77), (h.n - 3, 3)):
+        dist, sums = g.apsp_block(first, count, policy="learned")
+        assert np.array_equal(dist, odist[first:first + count]), (first, count)
+        assert np.array_equal(sums, osums[first:first + count])
+    with pytest.raises(dawn.DawnError):
+        g.apsp_block(h.n - 1, 2)
+
+
+def test_minplus_sums_beyond_int32_are_unreachable(orc):
+    """Paths whose weight reaches INT32_MAX are not representable: the
+    single-source, batched and step-level min-plus engines all treat them as
+    absent (no wrap to negative distances), as the oracle does.  Path
+    0-1-2-3-4 with weights 1.5e9, 6e8, 1, 1.5e9 plus a side branch 0-5 of
+    2.1e9."""
+    u = np.array([0, 1, 2, 3, 0], np.int32)
+    v = np.array([1, 2, 3, 4, 5], np.int32)
+    w = np.array([1500000000, 600000000, 1, 1500000000, 2100000000], np.int32)
+    h = dawn.HostGraph.from_edges(6, u, v, w)
+    g = dawn.Graph(h)
+    want = {s: orc.dijkstra(h.row_ptr, h.col, h.weight, s, overflow_ok=True) for s in range(6)}
+    assert list(want[0]) == [0, 1500000000, 2100000000, 2100000001, INF, 2100000000]
+    for s in range(6):
+        assert orc.certify(h.row_ptr, h.col, h.weight, s, want[s])
+        for p in ("fixed", "push", "pull", "bitmap"):
+            assert np.array_equal(g.sssp(s, policy=p), want[s]), (s, p)
+    srcs = np.arange(6, dtype=np.int32)
+    dist, sums = g.mssp(srcs)
+    assert np.array_equal(dist, np.stack([want[s] for s in range(6)]))
+    assert list(sums) == [int(want[s][want[s] != INF].astype(np.int64).sum()) for s in range(6)]
+    _, csums, _ = g.closeness(srcs)
+    assert np.array_equal(csums, sums)

@@ -81,3 +81,18 @@ def test_step_rejects_host_pointers():
     ch = dawn.DeviceArray(1, np.int64)
     with pytest.raises(dawn.DawnError):
         g.step(host, dev, dist, ch, level=1)
+
+
+def test_step_minplus_large_weights_do_not_wrap(orc):
+    """Min-plus steps with weights near 2^31: a candidate sum at or above
+    INT32_MAX is ignored (the vertex stays unreached), never wrapped."""
+    u = np.array([0, 1, 2, 3], np.int32)
+    v = np.array([1, 2, 3, 4], np.int32)
+    w = np.array([1500000000, 600000000, 1, 1500000000], np.int32)
+    h = dawn.HostGraph.from_edges(5, u, v, w)
+    g = dawn.Graph(h)
+    want = orc.dijkstra(h.row_ptr, h.col, h.weight, 0, overflow_ok=True)
+    assert list(want) == [0, 1500000000, 2100000000, 2100000001, INF]
+    for modes in (("push",), ("pull",)):
+        d, _ = run_steps(g, h.n, 0, modes)
+        assert np.array_equal(d, want), modes

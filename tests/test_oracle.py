@@ -248,3 +248,69 @@ def test_planted_oracle_mistakes_fail_pins(orc):
     bad2 = good.copy()
     bad2[reached[1]] = INF  # a dropped vertex
     assert not np.array_equal(bad2, ref)
+
+
+# ---- shortest-path certificate -------------------------------------------------
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_certify_accepts_distances_and_rejects_planted_mistakes(orc, seed):
+    """The certificate accepts scipy's distances (unit and positive int
+    weights, undirected and directed) and rejects every planted mistake: one
+    entry off by one either way, an unreached vertex marked reached, a
+    reached one marked unreached, a wrong source entry, and a consistent
+    shift of a whole subtree (a valid-looking potential that is not tight)."""
+    rng = np.random.default_rng(seed)
+    n = 400
+    u = rng.integers(0, n, 1500).astype(np.int32)
+    v = rng.integers(0, n, 1500).astype(np.int32)
+    w = rng.integers(1, 60, 1500).astype(np.int32)
+    for sym in (True, False):
+        for weighted in (False, True):
+            h = dawn.HostGraph.from_edges(n, u, v, w if weighted else None, symmetrize=sym)
+            s = int(np.argmax(np.diff(h.row_ptr)))
+            d = scipy_dist(h, s, weighted).astype(np.int32)
+            ww = h.weight if weighted else None
+            assert orc.certify(h.row_ptr, h.col, ww, s, d)
+            reached = np.flatnonzero((d != INF) & (np.arange(n) != s))
+            unreached = np.flatnonzero(d == INF)
+            x = int(reached[len(reached) // 2])
+            for delta in (-1, 1):
+                bad = d.copy()
+                bad[x] += delta
+                assert not orc.certify(h.row_ptr, h.col, ww, s, bad), (sym, weighted, delta)
+            bad = d.copy()
+            bad[x] = INF
+            assert not orc.certify(h.row_ptr, h.col, ww, s, bad)
+            bad = d.copy()
+            bad[s] = 1
+            assert not orc.certify(h.row_ptr, h.col, ww, s, bad)
+            if len(unreached):
+                bad = d.copy()
+                bad[unreached[0]] = int(d[reached].max()) + 1
+                assert not orc.certify(h.row_ptr, h.col, ww, s, bad)
+            # Every reached vertex other than the source shifted up by the
+            # same amount: edge inequalities still hold, tightness fails at
+            # the source's neighbours.
+            bad = d.copy()
+            bad[reached] += 1
+            assert not orc.certify(h.row_ptr, h.col, ww, s, bad)
+
+
+def test_certify_path_sums_beyond_int32_are_absent(orc):
+    """Path 0-1-2-3 with weights 1.5e9: vertex 2 (3e9) and 3 are unreachable
+    in int32, so the certificate takes [0, 1.5e9, INF, INF] and rejects a
+    wrapped negative value."""
+    h = dawn.HostGraph.from_edges(4, np.array([0, 1, 2], np.int32), np.array([1, 2, 3], np.int32),
+                                  np.array([1500000000] * 3, np.int32))
+    good = np.array([0, 1500000000, INF, INF], np.int32)
+    assert orc.certify(h.row_ptr, h.col, h.weight, 0, good)
+    wrapped = good.copy()
+    wrapped[2] = np.int32(-1294967296)
+    assert not orc.certify(h.row_ptr, h.col, h.weight, 0, wrapped)
+    for row in (orc.dijkstra(h.row_ptr, h.col, h.weight, 0, overflow_ok=True),
+                orc.sssp(h.row_ptr, h.col, h.weight, 0, overflow_ok=True),
+                orc.sssp(h.row_ptr, h.col, h.weight, 0, dense=False, overflow_ok=True),
+                orc.mssp(h.row_ptr, h.col, h.weight, [0], overflow_ok=True)[0][0]):
+        assert np.array_equal(row, good)
+    with pytest.raises(ValueError):
+        orc.sssp(h.row_ptr, h.col, h.weight, 0)
