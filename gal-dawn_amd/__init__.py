@@ -30,6 +30,7 @@ INF = 2147483647
 OK = 0
 ERR_ISOLATED_SOURCE = -2
 ERR_NO_DEVICE = -3
+ERR_SCRATCH_LIMIT = -11
 
 MODE_PUSH, MODE_PULL, MODE_BITMAP = 0, 1, 2
 MODE_NAMES = ("push", "pull", "bitmap")
@@ -127,6 +128,7 @@ def _sig(lib):
         "dawn_graph_info": (c.c_int, [P, _I32P, _I64P, _I32P, _I32P]),
         "dawn_bfs": (c.c_int, [P, c.c_int32, P, c.c_int32, c.POINTER(Stats)]),
         "dawn_sssp": (c.c_int, [P, c.c_int32, P, c.c_int32, c.POINTER(Stats)]),
+        "dawn_bfs_noalloc": (c.c_int, [P, c.c_int32, P, c.c_int32, c.POINTER(Stats)]),
         "dawn_mssp": (c.c_int, [P, _I32P, c.c_int32, P, _I64P, c.c_int32, c.POINTER(Stats)]),
         "dawn_apsp_block": (c.c_int, [P, c.c_int32, c.c_int32, P, _I64P, c.c_int32, c.POINTER(Stats)]),
         "dawn_closeness": (c.c_int, [P, _I32P, c.c_int32, _F64P, _I64P, _F32P, c.c_int32, c.POINTER(Stats)]),
@@ -444,6 +446,15 @@ class Graph:
         st = Stats()
         _check(lib().dawn_bfs(self._h, int(source), _out_arg(dist), _policy(policy),
                               ctypes.byref(st)), "dawn_bfs")
+        return (dist, st.as_dict()) if stats else dist
+
+    def bfs_noalloc(self, source: int, policy="fixed", out=None, stats: bool = False):
+        """dawn_bfs without device allocation (fixed scratch region; graphs
+        of at most 4,194,304 vertices)."""
+        dist = np.empty(self.n, np.int32) if out is None else out
+        st = Stats()
+        _check(lib().dawn_bfs_noalloc(self._h, int(source), _out_arg(dist), _policy(policy),
+                                      ctypes.byref(st)), "dawn_bfs_noalloc")
         return (dist, st.as_dict()) if stats else dist
 
     def sssp(self, source: int, policy="fixed", out=None, stats: bool = False):

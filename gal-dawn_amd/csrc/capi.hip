@@ -65,6 +65,16 @@ const float* sched_mlp_device(int device) {
   return net.params;
 }
 
+const float* sched_mlp_device_if_present(int device, const float** norm) {
+  std::lock_guard<std::mutex> l(g_sched_lock);
+  *norm = nullptr;
+  if (!g_have_net) return nullptr;
+  auto it = g_nets.find(device);
+  if (it == g_nets.end() || !it->second.params) return nullptr;
+  *norm = it->second.norm;
+  return it->second.params;
+}
+
 const float* sched_norm_device(int device) {
   if (!sched_mlp_device(device)) return nullptr;
   std::lock_guard<std::mutex> l(g_sched_lock);

@@ -26,6 +26,8 @@ constexpr int32_t kInf = DAWN_INF;
 // Records the last HIP error text (per thread) for diagnostics.
 void set_hip_error(hipError_t e, const char* what, const char* file, int line);
 const char* last_error();
+// Records a failure text that is not a HIP error (per thread).
+void set_error(const char* text);
 
 #define DAWN_HIP(x)                                                  \
   do {                                                               \

@@ -32,6 +32,11 @@ void set_hip_error(hipError_t e, const char* what, const char* file, int line) {
 
 const char* last_error() { return g_last_error.c_str(); }
 
+void set_error(const char* text) {
+  g_last_error = text;
+  if (std::getenv("DAWN_VERBOSE")) std::fprintf(stderr, "dawn: %s\n", text);
+}
+
 bool is_device_pointer(const void* p) {
   if (!p) return false;
   hipPointerAttribute_t attr;

@@ -241,6 +241,7 @@ const char* dawn_status_string(int status) {
     case DAWN_ERR_OVERFLOW: return "size exceeds int32 CSR offsets";
     case DAWN_ERR_UNSUPPORTED: return "unsupported";
     case DAWN_ERR_NOT_WEIGHTED: return "graph has no weights";
+    case DAWN_ERR_SCRATCH_LIMIT: return "graph exceeds the fixed scratch region";
     default: return "unknown status";
   }
 }

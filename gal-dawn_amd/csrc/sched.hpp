@@ -215,5 +215,8 @@ __device__ inline void sched_decide(SchedState* sp, const LevelCounters* ring,
 // Host-side view of the installed scheduler network (sched.cpp).
 const float* sched_mlp_device(int device);
 const float* sched_norm_device(int device);
+// The network and normalisation already on `device`, or null: never
+// allocates (the device copy is made on first use by sched_mlp_device).
+const float* sched_mlp_device_if_present(int device, const float** norm);
 
 }  // namespace dawn

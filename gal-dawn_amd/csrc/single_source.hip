@@ -22,8 +22,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <map>
+#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "sched.hpp"
@@ -675,8 +678,9 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
     reach_kernel<<<grid_for(g->n), kBlock, 0, s>>>(ws->dist.p, g->row_ptr, g->n, ws->reach.p);
     unsigned long long rr[2];
     DAWN_HIP(hipMemcpyAsync(rr, ws->reach.p, sizeof rr, hipMemcpyDeviceToHost, s));
-    std::vector<int32_t> modes(std::max(levels, 1));
-    int nrec = std::min(levels, kTraceMax);
+    // Trace entries 1..levels hold the modes of the expanding steps.
+    std::vector<int32_t> modes(levels + 1);
+    const int nrec = std::min(levels + 1, kTraceMax);
     if (nrec > 0)
       DAWN_HIP(hipMemcpyAsync(modes.data(), ws->trace_mode.p, sizeof(int32_t) * nrec,
                               hipMemcpyDeviceToHost, s));
@@ -718,4 +722,167 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
   return DAWN_OK;
 }
 
+// ---- allocation-free BOVM traversal (dawn_bfs_noalloc) --------------------
+//
+// For callers that cannot allocate device memory (serving loops, harnesses
+// that forbid hipMalloc): the single-source BOVM traversal on a fixed
+// scratch region instead of the graph's workspace.  The region is the
+// file-scope device array `scratch` below, sized for graphs of up to
+// kNoAllocMaxN vertices (66 MB); the code object carries it, so each GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never passes the array's name to a kernel (in host code it is a stand-in,
+// not a device address): once per device, with that device current, the
+// first call takes its device address from hipGetSymbolAddress and keeps it
+// per device; every kernel argument and copy uses that address.
+//
+// Every call re-initialises what it reads (fill and seed kernels, a zeroing
+// kernel for the reach counters).  Calls on one GPU share the array, so the
+// calls of a process are serialised per device by a lock (two calls that
+// overlapped on one GPU would race).  The step loop is launched directly,
+// K decide+expand pairs between reads of the done flag: instantiating a
+// captured graph would allocate.  The scheduler network is used only if it
+// is already on the device (installing it allocates); otherwise the
+// learned policy falls back to the fixed rule, as when no network is set.
+
+constexpr int32_t kNoAllocMaxN = 1 << 22;
+constexpr int kNoAllocMaxDevices = 64;
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr int64_t kNoAllocWords = ((int64_t)kNoAllocMaxN + 63) / 64 * 2;
+constexpr size_t kNaDist = 0;
+constexpr size_t kNaQ = align256(kNaDist + sizeof(int32_t) * (size_t)kNoAllocMaxN);
+constexpr size_t kNaBm = align256(kNaQ + sizeof(int32_t) * 3 * (size_t)kNoAllocMaxN);
+constexpr size_t kNaRing = align256(kNaBm + sizeof(uint32_t) * 3 * (size_t)kNoAllocWords);
+constexpr size_t kNaState = align256(kNaRing + sizeof(LevelCounters) * kRing);
+constexpr size_t kNaFeat = align256(kNaState + sizeof(SchedState));
+constexpr size_t kNaMode = align256(kNaFeat + sizeof(float) * 8 * (size_t)kTraceMax);
+constexpr size_t kNaReach = align256(kNaMode + sizeof(int32_t) * (size_t)kTraceMax);
+constexpr size_t kNoAllocBytes = align256(kNaReach + 2 * sizeof(unsigned long long));
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_noalloc_lock[kNoAllocMaxDevices];
+unsigned char* g_noalloc_base[kNoAllocMaxDevices] = {};  // device address of `scratch`
+
+__global__ void zero_u64_kernel(unsigned long long* p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0ull;
+}
+
+}  // namespace
+
+int run_bfs_noalloc(dawn_graph* g, int32_t source, int32_t* dist_out, int32_t policy,
+                    dawn_stats* stats) {
+  if (!g || source < 0 || source >= g->n) return DAWN_ERR_INVALID_ARG;
+  if (g->n > kNoAllocMaxN) {
+    set_error("dawn_bfs_noalloc: the graph has more vertices than the scratch region holds "
+              "(at most 4194304)");
+    return DAWN_ERR_SCRATCH_LIMIT;
+  }
+  if (g->device < 0 || g->device >= kNoAllocMaxDevices) return DAWN_ERR_UNSUPPORTED;
+  if (g->h_row_ptr[source] == g->h_row_ptr[source + 1]) return DAWN_ERR_ISOLATED_SOURCE;
+  std::lock_guard<std::mutex> device_guard(g_noalloc_lock[g->device]);
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  unsigned char* base = g_noalloc_base[g->device];
+  if (!base) {
+    DAWN_HIP(hipGetSymbolAddress((void**)&base, HIP_SYMBOL(scratch)));
+    if (!base) return DAWN_ERR_HIP;
+    g_noalloc_base[g->device] = base;
+  }
+  hipStream_t s = g->stream;
+
+  SsArgs a{};
+  a.n = g->n;
+  a.source = source;
+  a.words = ((int64_t)g->n + 63) / 64 * 2;
+  a.row_ptr = g->row_ptr;
+  a.col = g->col;
+  a.w = g->weight;
+  a.in_row_ptr = g->in_row_ptr;
+  a.in_col = g->in_col;
+  a.in_w = g->in_weight;
+  a.dist = (int32_t*)(base + kNaDist);
+  a.q = (int32_t*)(base + kNaQ);
+  a.bm = (uint32_t*)(base + kNaBm);
+  a.ring = (LevelCounters*)(base + kNaRing);
+  a.st = (SchedState*)(base + kNaState);
+  unsigned long long* reach = (unsigned long long*)(base + kNaReach);
+
+  SchedParams p{};
+  p.n = g->n;
+  p.m = g->m;
+  p.policy = policy;
+  p.alpha = 15;
+  p.beta = 18;
+  p.batched = 0;
+  p.allow_pull = 1;
+  p.avg_degree = g->n ? (float)((double)g->m / g->n) : 0.f;
+  p.max_degree = (float)g->max_out_degree;
+  p.mlp = sched_mlp_device_if_present(g->device, &p.mlp_norm);
+  SchedTrace tr{(float*)(base + kNaFeat), (int32_t*)(base + kNaMode)};
+
+  const unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock, 4096);
+  const int K = g->knobs.ss_chunk > 0 ? g->knobs.ss_chunk : (g->n < (1 << 20) ? 8 : 16);
+  const auto t0 = std::chrono::steady_clock::now();
+  ss_fill_kernel<<<grid_for(std::max<int64_t>(3 * a.words, g->n)), kBlock, 0, s>>>(a, 0);
+  bfs_seed_kernel<<<1, 64, 0, s>>>(a, policy);
+  DAWN_HIP(hipGetLastError());
+  SchedState h_state{};
+  int64_t launched = 0;
+  const int64_t limit = (int64_t)g->n + 2;
+  for (;;) {
+    for (int i = 0; i < K; i++) {
+      ss_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+      bfs_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+    }
+    DAWN_HIP(hipGetLastError());
+    launched += K;
+    DAWN_HIP(hipMemcpyAsync(&h_state, a.st, sizeof(SchedState), hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    if (h_state.done || launched > limit) break;
+  }
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (!h_state.done) return DAWN_ERR_HIP;  // step bound exceeded
+
+  if (stats) {
+    const int levels = h_state.levels;
+    zero_u64_kernel<<<1, 64, 0, s>>>(reach, 2);
+    reach_kernel<<<grid_for(g->n), kBlock, 0, s>>>(a.dist, g->row_ptr, g->n, reach);
+    DAWN_HIP(hipGetLastError());
+    unsigned long long rr[2];
+    DAWN_HIP(hipMemcpyAsync(rr, reach, sizeof rr, hipMemcpyDeviceToHost, s));
+    // Trace entries 1..levels hold the modes of the expanding steps.
+    std::vector<int32_t> modes(levels + 1);
+    const int nrec = std::min(levels + 1, kTraceMax);
+    if (nrec > 0)
+      DAWN_HIP(hipMemcpyAsync(modes.data(), tr.mode, sizeof(int32_t) * nrec,
+                              hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    stats->device_ms += ms;
+    stats->vertices_reached += (int64_t)rr[0];
+    stats->edges_reached += (int64_t)rr[1];
+    stats->batches += 1;
+    for (int st = 1; st <= levels && st < kTraceMax; st++) {
+      stats->steps += 1;
+      const int md = modes[st];
+      if (md >= 0 && md < DAWN_NUM_MODES) stats->mode_steps[md] += 1;
+    }
+  }
+  if (dist_out) {
+    DAWN_HIP(hipMemcpyAsync(dist_out, a.dist, sizeof(int32_t) * g->n, hipMemcpyDefault, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+  }
+  return DAWN_OK;
+}
+
 }  // namespace dawn
+
+extern "C" int dawn_bfs_noalloc(const dawn_graph* g, int32_t source, int32_t* dist,
+                                int32_t policy, dawn_stats* stats) {
+  return dawn::run_bfs_noalloc(const_cast<dawn_graph*>(g), source, dist, policy, stats);
+}
+

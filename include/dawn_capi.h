@@ -37,7 +37,8 @@ enum {
   DAWN_ERR_FORMAT = -7,      /* not a MatrixMarket coordinate file / bad weight */
   DAWN_ERR_OVERFLOW = -8,    /* sizes beyond int32 CSR offsets */
   DAWN_ERR_UNSUPPORTED = -9,
-  DAWN_ERR_NOT_WEIGHTED = -10
+  DAWN_ERR_NOT_WEIGHTED = -10,
+  DAWN_ERR_SCRATCH_LIMIT = -11  /* graph larger than the fixed scratch region */
 };
 
 /* Step modes of the frontier engine.  Every mode produces identical
@@ -152,6 +153,23 @@ int dawn_bfs(const dawn_graph* g, int32_t source, int32_t* dist,
  * (src/algorithm/cpu/sssp.cpp:9-21). */
 int dawn_sssp(const dawn_graph* g, int32_t source, int32_t* dist,
               int32_t policy, dawn_stats* stats);
+
+/* dawn_bfs without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and statuses as
+ * dawn_bfs, but the traversal state lives in a fixed scratch region that
+ * the library's code object carries (66 MB per GPU, held until the process
+ * ends) instead of the graph's workspace, so graphs of at most 4,194,304
+ * vertices: larger ones return DAWN_ERR_SCRATCH_LIMIT.  The call allocates
+ * nothing (no hipMalloc / hipFree, no captured graph), on the graph's
+ * device and stream, and returns when the traversal is done.  The learned
+ * policy uses the scheduler network only if it is already on the device
+ * (another entry point has used it there); otherwise the fixed rule.
+ * stats->device_ms is host wall time of the traversal.  All calls on one
+ * GPU share its region: the calls of a process are serialised per device
+ * (calls that overlapped on one GPU would race). */
+int dawn_bfs_noalloc(const dawn_graph* g, int32_t source, int32_t* dist,
+                     int32_t policy, dawn_stats* stats);
 
 /* Multi-source shortest paths.  Unweighted graphs run 64*W sources per
  * bit-packed batch (W = the lane-words knob, up to 16: 1024 sources);
