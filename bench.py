@@ -1,9 +1,11 @@
 """Benchmark: unweighted MSSP on RMAT scale 24, edge factor 16 (symmetrized,
 deduplicated, seed 99), 4096 uniform sources (seed 99), on one MI355X.
 
-A step = one MSSP pass over all sources: bit-packed batches of 64*W sources
-(W = the lane_words knob, 16 from the committed tune cache: 4 batches of
-1024), two batches in flight, every batch's int32 distance rows written to
+A step = one MSSP pass over all sources: the sources with out-edges in
+bit-packed batches of 64*W (W = the lane_words knob, 16 from the committed
+tune cache: 1024 per batch; isolated sources take no lane, as the reference
+skips them, and their rows -- INF but 0 at the source -- are written by a
+fill), two batches in flight, every source's int32 distance row written to
 HBM.  Reported value: GTEPS of the learned schedule, whole job, inputs
 resident in HBM.  Also: sources/s, the fixed alpha/beta schedule's numbers,
 the closeness of the same sources (sums only), the per-step oracle-best
@@ -30,6 +32,9 @@ import __graft_entry__  # noqa: E402
 METRIC = "GTEPS + MSSP sources/s on 1 MI355X, learned vs fixed schedule, bit-exact dists"
 HBM_PEAK_GBS = 8000.0
 TUNE_PATH = os.path.join(ROOT, "gal-dawn_amd", "tune_cache.txt")
+# The engine's knobs before any tuning (dawn::TuneKnobs defaults).
+DEFAULT_KNOBS = {"dense_blocks": 2048, "heavy_degree": 32, "pull_chunk": 256, "ss_chunk": 0,
+                 "pull_variant": 0, "lane_words": 16, "hub_order": 1}
 
 
 def log(msg):
@@ -158,10 +163,14 @@ def host_cpu():
             "omp_num_threads_env": env or None, "cpu_model": model}
 
 
-def run_suite(dawn, orc, steps):
+SUITE = ("er65k", "grid2048", "rmat20", "uneven512")
+
+
+def run_suite(dawn, orc, steps, only=SUITE):
     """The other configurations, one JSON line each (GPU device time, best of
     `steps` runs after one warm-up; rows checked against the oracle on a
-    sample; the reference's CPU MSSP timed beside where it is correct)."""
+    sample; the reference's CPU MSSP timed beside where it is correct).
+    only: the configurations to run."""
     wl = importlib.import_module("gal_dawn_amd.workloads")
     threads = host_cpu()["threads"]
 
@@ -169,92 +178,105 @@ def run_suite(dawn, orc, steps):
         fn()
         return min(fn()["device_ms"] for _ in range(steps))
 
-    # er65k: single-source BFS (source 0) and 65-source MSSP.
-    h, src = wl.er65k()
-    g = dawn.Graph(h)
-    bfs_ms = best(lambda: g.bfs(int(src[0]), stats=True)[1])
-    ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
-    dist, _ = g.mssp(src)
-    ok = bool(np.array_equal(dist, orc.mssp(h.row_ptr, h.col, None, src)[0]))
-    t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src, threads)
-    # The other CPU mode: single-source dense OpenMP BOVM (the oracle's port
-    # of BFS_CPU::BFS, src/algorithm/cpu/bfs.cpp:23-77), best of 3.
-    t_dense = min(orc.time_bfs_dense(h.row_ptr, h.col, int(src[0]), threads) for _ in range(3))
-    print(json.dumps({"config": "er65k: ER n=65,536 deg 8 seed 1, source 0 + 64", "bfs_ms": bfs_ms,
-                      "mssp_ms": ms_ms, "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
-                      "cpu_reference_sources_per_s": len(src) / t_cpu, "cpu_threads": threads,
-                      "cpu_dense_bfs_ms": 1e3 * t_dense, "rows_bit_exact": ok}), flush=True)
-    g.close()
-
-    # grid2048: weighted single-source SSSP (int32 min-plus) and 16-source MSSP.
-    h, src = wl.grid2048()
-    g = dawn.Graph(h)
-    sssp_ms = best(lambda: g.sssp(int(src[0]), stats=True)[1])
-    _, st = g.sssp(int(src[0]), stats=True)
-    ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
-    ms_steps = g.mssp(src, want_dist=False, stats=True)[2]["steps"]
-    singles_ms = sum(g.sssp(int(x), stats=True)[1]["device_ms"] for x in src)
-    ok = bool(np.array_equal(g.sssp(int(src[0])), orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[0]))))
-    rows, _ = g.mssp(src[:2])
-    ok = ok and all(np.array_equal(rows[i], orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[i])))
-                    for i in range(2))
-    print(json.dumps({"config": "grid2048: 2048^2 grid + 1% long-range, int32 w in [1,1000], seed 2",
-                      "sssp_ms": sssp_ms, "sssp_steps": st["steps"], "mssp16_ms": ms_ms,
-                      "mssp16_steps": ms_steps, "sssp_x16_ms": singles_ms,
-                      "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
-                      "rows_bit_exact_vs_dijkstra": ok}), flush=True)
-    g.close()
-
-    # rmat20: 4096-source MSSP.
-    h, src = wl.rmat20()
-    g = dawn.Graph(h)
-    runs = [g.mssp(src, want_dist=False, stats=True)[2] for _ in range(steps + 1)][1:]
-    st = min(runs, key=lambda r: r["device_ms"])
-    chk = src[:8]
-    ok = bool(np.array_equal(g.mssp(chk)[0], orc.mssp(h.row_ptr, h.col, None, chk)[0]))
-    t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src[:2 * threads], threads)
-    print(json.dumps({"config": "rmat20: RMAT scale 20 ef 16 seed 3, 4096 sources",
-                      "mssp_ms": st["device_ms"], "gteps": st["edges_reached"] / st["device_ms"] / 1e6,
-                      "sources_per_s": len(src) / (st["device_ms"] / 1e3),
-                      "cpu_reference_sources_per_s": 2 * threads / t_cpu, "cpu_threads": threads,
-                      "rows_bit_exact_8": ok}), flush=True)
-    # All-vertex closeness (every vertex a source; sums only, no rows),
-    # checked on sampled vertices against the oracle's rows.
-    everyone = np.arange(h.n, dtype=np.int32)
-    _, _, _, cst = g.closeness(everyone, policy="learned", stats=True)
-    t0 = time.time()
-    cc, sums, ref_int, cst = g.closeness(everyone, policy="learned", stats=True)
-    wall = time.time() - t0
-    pick = np.linspace(0, h.n - 1, 6).astype(np.int32)
-    odist = orc.mssp(h.row_ptr, h.col, None, pick)[0]
-    cok = all(orc.closeness(odist[i]) == (cc[v], sums[v], ref_int[v]) for i, v in enumerate(pick))
-    print(json.dumps({"config": "rmat20: closeness of all 1,048,576 vertices (sums only)",
-                      "device_ms": cst["device_ms"], "wall_s": wall,
-                      "sources_per_s": h.n / (cst["device_ms"] / 1e3),
-                      "closeness_bit_exact_sample": bool(cok)}), flush=True)
-    g.close()
-
-    # uneven512: 512 graphs x 8 sources in one persistent launch.
-    for weighted in (False, True):
-        hs, src, _ = wl.uneven(weighted=weighted)
-        gs = [dawn.Graph(x) for x in hs]
-        dawn.batch_mssp(gs, src, 8, want_dist=False)
-        t = min(dawn.batch_mssp(gs, src, 8, want_dist=False)[1]["device_ms"] for _ in range(steps))
-        rows, _ = dawn.batch_mssp(gs, src, 8)
-        ok = True
-        for i in range(0, 512, 97):
-            x = hs[i]
-            if weighted:
-                ref_rows = [orc.dijkstra(x.row_ptr, x.col, x.weight, int(s)) for s in src[8 * i: 8 * i + 8]]
-            else:
-                ref_rows = orc.mssp(x.row_ptr, x.col, None, src[8 * i: 8 * i + 8])[0]
-            ok = ok and all(np.array_equal(rows[i][k], ref_rows[k]) for k in range(8))
-        print(json.dumps({"config": f"uneven512: 512 graphs (1k-130k vertices), 8 sources each, "
-                                    f"{'int32 weights' if weighted else 'unweighted'}, seed 4",
-                          "batch_ms": t, "sources_per_s": 512 * 8 / (t / 1e3),
-                          "vertices": int(sum(x.n for x in hs)), "rows_bit_exact_sample": ok}), flush=True)
-        for x in gs:
-            x.close()
+    if "er65k" in only:
+        # er65k: single-source BFS (source 0) and 65-source MSSP.
+        h, src = wl.er65k()
+        g = dawn.Graph(h)
+        bfs_ms = best(lambda: g.bfs(int(src[0]), stats=True)[1])
+        ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
+        dist, _ = g.mssp(src)
+        ok = bool(np.array_equal(dist, orc.mssp(h.row_ptr, h.col, None, src)[0]))
+        t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src, threads)
+        # The other CPU mode: single-source dense OpenMP BOVM (the oracle's port
+        # of BFS_CPU::BFS, src/algorithm/cpu/bfs.cpp:23-77), best of 3.
+        t_dense = min(orc.time_bfs_dense(h.row_ptr, h.col, int(src[0]), threads) for _ in range(3))
+        print(json.dumps({"config": "er65k: ER n=65,536 deg 8 seed 1, source 0 + 64", "bfs_ms": bfs_ms,
+                          "mssp_ms": ms_ms, "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
+                          "cpu_reference_sources_per_s": len(src) / t_cpu, "cpu_threads": threads,
+                          "cpu_dense_bfs_ms": 1e3 * t_dense, "rows_bit_exact": ok}), flush=True)
+        g.close()
+    if "grid2048" in only:
+        # grid2048: weighted single-source SSSP (int32 min-plus) and 16-source MSSP.
+        h, src = wl.grid2048()
+        g = dawn.Graph(h)
+        sssp_ms = best(lambda: g.sssp(int(src[0]), stats=True)[1])
+        _, st = g.sssp(int(src[0]), stats=True)
+        ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
+        ms_steps = g.mssp(src, want_dist=False, stats=True)[2]["steps"]
+        singles_ms = sum(g.sssp(int(x), stats=True)[1]["device_ms"] for x in src)
+        ok = bool(np.array_equal(g.sssp(int(src[0])), orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[0]))))
+        rows, _ = g.mssp(src[:2])
+        ok = ok and all(np.array_equal(rows[i], orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[i])))
+                        for i in range(2))
+        # Algorithmic bytes (Dijkstra-optimal lower bound): offsets, columns
+        # and weights once, one 4-byte atomic per edge, distances read and
+        # written once -- per source; the batch shares the CSR reads.
+        n, m = h.n, h.m
+        b_one = 4 * (n + 1) + 8 * m + 4 * m + 8 * n
+        b_batch = 4 * (n + 1) + 8 * m + len(src) * (4 * m + 8 * n)
+        fe = st.get("frontier_edges")
+        print(json.dumps({"config": "grid2048: 2048^2 grid + 1% long-range, int32 w in [1,1000], seed 2",
+                          "sssp_ms": sssp_ms, "sssp_steps": st["steps"],
+                          "sssp_mode_steps": st["mode_steps"],
+                          "sssp_relaxations_per_edge": fe / m if fe else None,
+                          "sssp_roofline_frac": b_one / (sssp_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "mssp16_ms": ms_ms, "mssp16_steps": ms_steps,
+                          "mssp16_roofline_frac": b_batch / (ms_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "sssp_x16_ms": singles_ms,
+                          "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
+                          "rows_bit_exact_vs_dijkstra": ok}), flush=True)
+        g.close()
+    if "rmat20" in only:
+        # rmat20: 4096-source MSSP.
+        h, src = wl.rmat20()
+        g = dawn.Graph(h)
+        runs = [g.mssp(src, want_dist=False, stats=True)[2] for _ in range(steps + 1)][1:]
+        st = min(runs, key=lambda r: r["device_ms"])
+        chk = src[:8]
+        ok = bool(np.array_equal(g.mssp(chk)[0], orc.mssp(h.row_ptr, h.col, None, chk)[0]))
+        t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src[:2 * threads], threads)
+        print(json.dumps({"config": "rmat20: RMAT scale 20 ef 16 seed 3, 4096 sources",
+                          "mssp_ms": st["device_ms"], "gteps": st["edges_reached"] / st["device_ms"] / 1e6,
+                          "sources_per_s": len(src) / (st["device_ms"] / 1e3),
+                          "cpu_reference_sources_per_s": 2 * threads / t_cpu, "cpu_threads": threads,
+                          "rows_bit_exact_8": ok}), flush=True)
+        # All-vertex closeness (every vertex a source; sums only, no rows),
+        # checked on sampled vertices against the oracle's rows.
+        everyone = np.arange(h.n, dtype=np.int32)
+        _, _, _, cst = g.closeness(everyone, policy="learned", stats=True)
+        t0 = time.time()
+        cc, sums, ref_int, cst = g.closeness(everyone, policy="learned", stats=True)
+        wall = time.time() - t0
+        pick = np.linspace(0, h.n - 1, 6).astype(np.int32)
+        odist = orc.mssp(h.row_ptr, h.col, None, pick)[0]
+        cok = all(orc.closeness(odist[i]) == (cc[v], sums[v], ref_int[v]) for i, v in enumerate(pick))
+        print(json.dumps({"config": "rmat20: closeness of all 1,048,576 vertices (sums only)",
+                          "device_ms": cst["device_ms"], "wall_s": wall,
+                          "sources_per_s": h.n / (cst["device_ms"] / 1e3),
+                          "closeness_bit_exact_sample": bool(cok)}), flush=True)
+        g.close()
+    if "uneven512" in only:
+        # uneven512: 512 graphs x 8 sources in one persistent launch.
+        for weighted in (False, True):
+            hs, src, _ = wl.uneven(weighted=weighted)
+            gs = [dawn.Graph(x) for x in hs]
+            dawn.batch_mssp(gs, src, 8, want_dist=False)
+            t = min(dawn.batch_mssp(gs, src, 8, want_dist=False)[1]["device_ms"] for _ in range(steps))
+            rows, _ = dawn.batch_mssp(gs, src, 8)
+            ok = True
+            for i in range(0, 512, 97):
+                x = hs[i]
+                if weighted:
+                    ref_rows = [orc.dijkstra(x.row_ptr, x.col, x.weight, int(s)) for s in src[8 * i: 8 * i + 8]]
+                else:
+                    ref_rows = orc.mssp(x.row_ptr, x.col, None, src[8 * i: 8 * i + 8])[0]
+                ok = ok and all(np.array_equal(rows[i][k], ref_rows[k]) for k in range(8))
+            print(json.dumps({"config": f"uneven512: 512 graphs (1k-130k vertices), 8 sources each, "
+                                        f"{'int32 weights' if weighted else 'unweighted'}, seed 4",
+                              "batch_ms": t, "sources_per_s": 512 * 8 / (t / 1e3),
+                              "vertices": int(sum(x.n for x in hs)), "rows_bit_exact_sample": ok}), flush=True)
+            for x in gs:
+                x.close()
 
 
 def main():
@@ -280,9 +302,9 @@ def main():
                     help="one learned-schedule pass only (for rocprofv3 runs)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the 1 GiB plain copy kernel (rocprofv3 traffic calibration)")
-    ap.add_argument("--suite", action="store_true",
-                    help="the other workloads (ER 65k, weighted grid 2048^2, RMAT-20, 512 uneven "
-                         "graphs), one JSON line each, instead of the headline line")
+    ap.add_argument("--suite", nargs="?", const="all", default=None,
+                    help="the other workloads (er65k, grid2048, rmat20, uneven512: all, or a "
+                         "comma-separated subset), one JSON line each, instead of the headline line")
     args = ap.parse_args()
     if args.summarize_rocprof:
         summary, table = summarize_rocprof(args.summarize_rocprof, os.path.join(ROOT, "profiles"))
@@ -305,7 +327,11 @@ def main():
     if args.suite:
         if os.path.exists(TUNE_PATH):
             dawn.tune_load(TUNE_PATH)
-        run_suite(dawn, __graft_entry__.load_oracle(), max(1, args.steps))
+        only = SUITE if args.suite == "all" else tuple(args.suite.split(","))
+        unknown = [x for x in only if x not in SUITE]
+        if unknown:
+            raise SystemExit(f"unknown suite workloads {unknown}; choose from {SUITE}")
+        run_suite(dawn, __graft_entry__.load_oracle(), max(1, args.steps), only)
         return
 
     t0 = time.time()
@@ -315,7 +341,8 @@ def main():
     log(f"graph RMAT scale {args.scale} ef {args.edge_factor}: n={n} m={m} built in {build_s:.1f}s")
     # Launch knobs measured by the autotuner on this graph class, committed
     # beside the package; graphs created afterwards start from them.
-    if os.path.exists(TUNE_PATH) and not args.retune:
+    # (Loaded with --retune too, so that a saved cache keeps the other classes.)
+    if os.path.exists(TUNE_PATH):
         dawn.tune_load(TUNE_PATH)
     t0 = time.time()
     g = dawn.Graph(h)
@@ -323,8 +350,10 @@ def main():
     src = dawn.gen_sources(n, args.sources, args.seed)
     tune_s = 0.0
     if args.retune:
-        # Sweep the launch knobs for this graph class (untimed set-up).
+        # Sweep the launch knobs for this graph class (untimed set-up),
+        # starting from the engine's defaults, not from the cached knobs.
         t0 = time.time()
+        g.set_knobs(**DEFAULT_KNOBS)
         dawn.tune(g)
         tune_s = time.time() - t0
         if args.tune_out:
@@ -401,9 +430,11 @@ def main():
                             "sources_per_s": args.sources / t, "sums_equal_mssp": True}
     log(f"closeness (sums only): {results['closeness']}")
 
-    # Per-step oracle-best schedule on the first 4 batches: every mode timed
-    # from identical state (dawn_sched_trace), then each policy's choices.
-    sub = src[:64 * W]  # one batch at the timed run's width
+    # Per-step oracle-best schedule on the first batch of the pass: every
+    # mode timed from identical state (dawn_sched_trace), then each policy's
+    # choices.  Isolated sources take no lane, so the first batch holds the
+    # first 64*W sources with edges.
+    sub = src[h.degrees()[src] > 0][:64 * W]
     regret = {}
     for pol in ("fixed", "learned"):
         modes, feat, times = g.sched_trace(sub, policy=pol)
@@ -460,9 +491,13 @@ def main():
         "avg_launch_ms": kern[dom]["avg_ms"],
         "bytes_per_launch": dom_bytes,
     }
-    b_batch = batch_bytes(n, m, L, W)
+    # The pass: every batch's traversal bytes (its rows excluded) plus one
+    # int32 row per source, isolated sources' rows included.
+    nb = len(levels)
+    pass_bytes = sum(batch_bytes(n, m, float(lv), W) - 256 * W * n for lv in levels) \
+        + 4 * n * args.sources
     lt = results["learned"]["device_ms_per_step"] / 1e3
-    batch_frac = (b_batch * (args.sources / (64 * W))) / lt / 1e9 / HBM_PEAK_GBS
+    batch_frac = pass_bytes / lt / 1e9 / HBM_PEAK_GBS
     log(f"kernels: {kern}; levels/batch {L:.2f}; dominant {dom}")
 
     # Bit-exact spot check against the CPU oracle on sampled sources.
@@ -524,7 +559,8 @@ def main():
         "config": {
             "workload": f"RMAT a=.57 b=.19 c=.19 scale {args.scale} ef {args.edge_factor} symmetrized "
                         f"dedup seed {args.seed}; unweighted MSSP of {args.sources} sources, {64 * W} "
-                        f"per batch bit-packed; every batch's int32 distance rows written to HBM",
+                        f"per batch bit-packed (isolated sources take no lane; their rows are a "
+                        f"fill); every source's int32 distance row written to HBM",
             "n": n, "m": m, "sources": args.sources, "parallelism": "single GPU",
         },
         "sources_per_s": results["learned"]["sources_per_s"],
@@ -534,6 +570,9 @@ def main():
         "schedule": sched,
         "roofline": roofline,
         "batch_roofline_frac": batch_frac,
+        "pass_bytes": pass_bytes,
+        "batches": nb,
+        "isolated_sources": int((h.degrees()[src] == 0).sum()),
         "lane_words": W,
         "kernels": kern,
         "levels_per_batch": L,

@@ -83,6 +83,7 @@ class Stats(ctypes.Structure):
         ("batches", ctypes.c_int64),
         ("skipped_sources", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("frontier_edges", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -94,6 +95,7 @@ class Stats(ctypes.Structure):
             "vertices_reached": self.vertices_reached,
             "batches": self.batches,
             "skipped_sources": self.skipped_sources,
+            "frontier_edges": self.frontier_edges,
         }
 
 
@@ -118,6 +120,7 @@ def _sig(lib):
         "dawn_gen_sources": (c.c_int, [c.c_int32, c.c_int32, c.c_uint64, _I32P]),
         "dawn_csr_from_edges": (c.c_int, [c.c_int32, c.c_int64, _I32P, _I32P, _I32P, c.c_int32, c.c_int32, c.POINTER(_Csr)]),
         "dawn_read_mtx": (c.c_int, [c.c_char_p, c.c_int32, c.POINTER(_Csr)]),
+        "dawn_read_mtx_as": (c.c_int, [c.c_char_p, c.c_int32, c.c_int32, c.POINTER(_Csr)]),
         "dawn_write_mtx": (c.c_int, [c.c_char_p, c.POINTER(_Csr)]),
         "dawn_csr_free": (None, [c.POINTER(_Csr)]),
         "dawn_read_list": (c.c_int, [c.c_char_p, c.POINTER(_I32P), _I64P]),
@@ -126,6 +129,15 @@ def _sig(lib):
         "dawn_graph_create": (c.c_int, [c.POINTER(_Csr), c.POINTER(P)]),
         "dawn_graph_destroy": (c.c_int, [P]),
         "dawn_graph_info": (c.c_int, [P, _I32P, _I64P, _I32P, _I32P]),
+        "dawn_graph_from_edges": (c.c_int, [c.c_int32, c.c_int64, P, P, P, c.c_int32, c.c_int32,
+                                            c.POINTER(P)]),
+        "dawn_graph_gen_er": (c.c_int, [c.c_int32, c.c_double, c.c_uint64, c.c_int32, c.c_int32,
+                                        c.c_int32, c.POINTER(P)]),
+        "dawn_graph_gen_rmat": (c.c_int, [c.c_int32, c.c_int32, c.c_double, c.c_double, c.c_double,
+                                          c.c_uint64, c.c_int32, c.c_int32, c.c_int32, c.POINTER(P)]),
+        "dawn_graph_gen_grid": (c.c_int, [c.c_int32, c.c_double, c.c_uint64, c.c_int32, c.c_int32,
+                                          c.c_int32, c.POINTER(P)]),
+        "dawn_graph_download": (c.c_int, [P, c.POINTER(_Csr)]),
         "dawn_bfs": (c.c_int, [P, c.c_int32, P, c.c_int32, c.POINTER(Stats)]),
         "dawn_sssp": (c.c_int, [P, c.c_int32, P, c.c_int32, c.POINTER(Stats)]),
         "dawn_bfs_noalloc": (c.c_int, [P, c.c_int32, P, c.c_int32, c.POINTER(Stats)]),
@@ -145,6 +157,11 @@ def _sig(lib):
         "dawn_sched_record": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, _I32P, c.c_int32,
                                         _I32P]),
         "dawn_sched_trace": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _I32P, _F32P, _F32P, c.c_int32, _I32P]),
+        "dawn_sched_set_mlp_single": (c.c_int, [_F32P, _F32P, _F32P]),
+        "dawn_sched_record_single": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _F32P, _F32P, _I32P,
+                                               c.c_int32, _I32P]),
+        "dawn_sched_trace_single": (c.c_int, [P, _I32P, c.c_int32, c.c_int32, _I32P, _F32P, _F32P,
+                                              c.c_int32, _I32P]),
         "dawn_mlp_forward": (c.c_int, [_F32P, _F32P, c.c_int32, _F32P]),
         "dawn_mlp_loss_grad": (c.c_int, [_F32P, _F32P, _F32P, c.c_int32, _F32P, _F32P]),
         "dawn_mlp_train": (c.c_int, [_F32P, _F32P, _F32P, c.c_int32, c.c_int32, c.c_int32, c.c_float, c.c_uint64, _F32P]),
@@ -263,8 +280,14 @@ class HostGraph:
                            what="dawn_csr_from_edges")
 
     @classmethod
-    def read_mtx(cls, path: str, weighted: bool = False) -> "HostGraph":
-        return cls._from_c(lib().dawn_read_mtx, path.encode(), int(weighted), what="dawn_read_mtx")
+    def read_mtx(cls, path: str, weighted: bool = False, directed: Optional[bool] = None) -> "HostGraph":
+        """MatrixMarket file; directed None follows the header's symmetry
+        field, True / False read it as the reference's directed / undirected
+        readers do."""
+        if directed is None:
+            return cls._from_c(lib().dawn_read_mtx, path.encode(), int(weighted), what="dawn_read_mtx")
+        return cls._from_c(lib().dawn_read_mtx_as, path.encode(), int(weighted), int(directed),
+                           what="dawn_read_mtx_as")
 
     @classmethod
     def from_csr(cls, row_ptr, col, weight=None, directed: bool = False) -> "HostGraph":
@@ -420,14 +443,73 @@ class DeviceArray:
 class Graph:
     """Device-resident graph (dawn_graph): CSR (+ CSC when directed)."""
 
-    def __init__(self, host: HostGraph):
+    def __init__(self, host: Optional[HostGraph] = None):
         self._h = ctypes.c_void_p()
+        if host is None:  # filled by a device-side builder
+            return
         _check(lib().dawn_graph_create(ctypes.byref(host._csr), ctypes.byref(self._h)),
                "dawn_graph_create")
         self.n = host.n
         self.m = host.m
         self.directed = host.directed
         self.weighted = host.weighted
+
+    # -- built on the device (edge list / generators -> CSR on the GPU) ----
+    @classmethod
+    def _built(cls, fn, *args, what) -> "Graph":
+        g = cls()
+        _check(fn(*args, ctypes.byref(g._h)), what)
+        n, m, d, w = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().dawn_graph_info(g._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(d),
+                                     ctypes.byref(w)), "dawn_graph_info")
+        g.n, g.m, g.directed, g.weighted = n.value, m.value, bool(d.value), bool(w.value)
+        return g
+
+    @classmethod
+    def from_edges_device(cls, n: int, u, v, w=None, symmetrize: bool = True,
+                          dedup: bool = True) -> "Graph":
+        """Edge list (numpy host arrays or device buffers) -> CSR built on the
+        device, identical to HostGraph.from_edges + Graph."""
+        keep = []  # host arrays stay alive through the call
+
+        def arg(a):
+            if a is None:
+                return None
+            if isinstance(a, DeviceArray):
+                return ctypes.c_void_p(a.data_ptr())
+            x = _i32(a)
+            keep.append(x)
+            return ctypes.c_void_p(x.ctypes.data)
+        pu, pv, pw = arg(u), arg(v), arg(w)
+        count = u.shape[0] if isinstance(u, DeviceArray) else len(u)
+        return cls._built(lib().dawn_graph_from_edges, n, count, pu, pv, pw, int(symmetrize),
+                          int(dedup), what="dawn_graph_from_edges")
+
+    @classmethod
+    def er_device(cls, n: int, avg_degree: float, seed: int, weighted: bool = False,
+                  wmin: int = 1, wmax: int = 1000) -> "Graph":
+        return cls._built(lib().dawn_graph_gen_er, n, avg_degree, seed, int(weighted), wmin, wmax,
+                          what="dawn_graph_gen_er")
+
+    @classmethod
+    def rmat_device(cls, scale: int, edge_factor: int, seed: int, a: float = 0.57,
+                    b: float = 0.19, c: float = 0.19, weighted: bool = False, wmin: int = 1,
+                    wmax: int = 1000) -> "Graph":
+        return cls._built(lib().dawn_graph_gen_rmat, scale, edge_factor, a, b, c, seed,
+                          int(weighted), wmin, wmax, what="dawn_graph_gen_rmat")
+
+    @classmethod
+    def grid_device(cls, side: int, extra_fraction: float, seed: int, weighted: bool = False,
+                    wmin: int = 1, wmax: int = 1000) -> "Graph":
+        return cls._built(lib().dawn_graph_gen_grid, side, extra_fraction, seed, int(weighted),
+                          wmin, wmax, what="dawn_graph_gen_grid")
+
+    def download(self) -> HostGraph:
+        """The device CSR as a host CSR (owned)."""
+        h = HostGraph()
+        _check(lib().dawn_graph_download(self._h, ctypes.byref(h._csr)), "dawn_graph_download")
+        h._owned = True
+        return h
 
     def close(self):
         if self._h and self._h.value and _lib is not None:
@@ -553,29 +635,33 @@ class Graph:
         k = np.array([cur[n] for n in self.KNOB_NAMES], np.int32)
         _check(lib().dawn_graph_set_knobs(self._h, _ptr(k, ctypes.c_int32)), "dawn_graph_set_knobs")
 
-    def sched_record(self, sources, repeats: int = 3, max_rows: int = 4096):
+    def sched_record(self, sources, repeats: int = 3, max_rows: int = 4096, single: bool = False):
+        """Per-step features, time of every mode and the fixed rule's choice
+        (batched engine, or with single=True the single-source engine)."""
         src = _i32(sources)
         feat = np.zeros((max_rows, SCHED_FEATURES), np.float32)
         times = np.zeros((max_rows, 3), np.float32)
         fixed = np.zeros(max_rows, np.int32)
         rows = ctypes.c_int32(0)
-        _check(lib().dawn_sched_record(self._h, _ptr(src, ctypes.c_int32), len(src), repeats,
-                                       _ptr(feat, ctypes.c_float), _ptr(times, ctypes.c_float),
-                                       _ptr(fixed, ctypes.c_int32), max_rows, ctypes.byref(rows)),
-               "dawn_sched_record")
+        fn = lib().dawn_sched_record_single if single else lib().dawn_sched_record
+        _check(fn(self._h, _ptr(src, ctypes.c_int32), len(src), repeats,
+                  _ptr(feat, ctypes.c_float), _ptr(times, ctypes.c_float),
+                  _ptr(fixed, ctypes.c_int32), max_rows, ctypes.byref(rows)),
+               "dawn_sched_record_single" if single else "dawn_sched_record")
         r = rows.value
         return feat[:r].copy(), times[:r].copy(), fixed[:r].copy()
 
-    def sched_trace(self, sources, policy="fixed", max_rows: int = 4096):
+    def sched_trace(self, sources, policy="fixed", max_rows: int = 4096, single: bool = False):
         src = _i32(sources)
         modes = np.zeros(max_rows, np.int32)
         feat = np.zeros((max_rows, SCHED_FEATURES), np.float32)
         times = np.zeros((max_rows, 3), np.float32)
         rows = ctypes.c_int32(0)
-        _check(lib().dawn_sched_trace(self._h, _ptr(src, ctypes.c_int32), len(src), _policy(policy),
-                                      _ptr(modes, ctypes.c_int32), _ptr(feat, ctypes.c_float),
-                                      _ptr(times, ctypes.c_float), max_rows, ctypes.byref(rows)),
-               "dawn_sched_trace")
+        fn = lib().dawn_sched_trace_single if single else lib().dawn_sched_trace
+        _check(fn(self._h, _ptr(src, ctypes.c_int32), len(src), _policy(policy),
+                  _ptr(modes, ctypes.c_int32), _ptr(feat, ctypes.c_float),
+                  _ptr(times, ctypes.c_float), max_rows, ctypes.byref(rows)),
+               "dawn_sched_trace_single" if single else "dawn_sched_trace")
         r = rows.value
         return modes[:r].copy(), feat[:r].copy(), times[:r].copy()
 
@@ -615,16 +701,19 @@ def tune_save(path: str):
     _check(lib().dawn_tune_save(path.encode()), "dawn_tune_save")
 
 
-def sched_set_mlp(params: Optional[np.ndarray], mean=None, inv_std=None):
+def sched_set_mlp(params: Optional[np.ndarray], mean=None, inv_std=None, single: bool = False):
+    """Installs (None: clears) the batched engine's scheduler network, or with
+    single=True the single-source engine's."""
+    fn = lib().dawn_sched_set_mlp_single if single else lib().dawn_sched_set_mlp
+    what = "dawn_sched_set_mlp_single" if single else "dawn_sched_set_mlp"
     if params is None:
-        _check(lib().dawn_sched_set_mlp(None, None, None), "dawn_sched_set_mlp")
+        _check(fn(None, None, None), what)
         return
     p = np.ascontiguousarray(params, np.float32)
     assert p.size == SCHED_PARAMS
     mu = np.ascontiguousarray(mean if mean is not None else np.zeros(8), np.float32)
     isd = np.ascontiguousarray(inv_std if inv_std is not None else np.ones(8), np.float32)
-    _check(lib().dawn_sched_set_mlp(_ptr(p, ctypes.c_float), _ptr(mu, ctypes.c_float),
-                                    _ptr(isd, ctypes.c_float)), "dawn_sched_set_mlp")
+    _check(fn(_ptr(p, ctypes.c_float), _ptr(mu, ctypes.c_float), _ptr(isd, ctypes.c_float)), what)
 
 
 def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:

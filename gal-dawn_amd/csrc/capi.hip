@@ -27,6 +27,9 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
                   int32_t* rows_out, int32_t policy);
 int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
                    double* kernel_ms, int64_t* launches, int32_t* levels_out);
+int run_ss_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
+                  float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
+                  int32_t* rows_out, int32_t policy);
 
 // ---- scheduler network store (one copy per device) -------------------------
 namespace {
@@ -35,20 +38,69 @@ struct SchedNet {
   float* params = nullptr;  // device
   float* norm = nullptr;    // device mean[8], inv_std[8]
 };
+// Batched engine's network (g_*), single-source engine's network (g_single_*).
 std::map<int, SchedNet> g_nets;
 std::vector<float> g_host_params, g_host_norm;
 bool g_have_net = false;
+std::map<int, SchedNet> g_single_nets;
+std::vector<float> g_single_params, g_single_norm;
+bool g_have_single = false;
 
-int install_on(int device, SchedNet& net) {
+int install_net(SchedNet& net, const std::vector<float>& params, const std::vector<float>& norm) {
   if (!net.params) DAWN_HIP(hipMalloc(&net.params, sizeof(float) * DAWN_SCHED_PARAMS));
   if (!net.norm) DAWN_HIP(hipMalloc(&net.norm, sizeof(float) * 16));
-  DAWN_HIP(hipMemcpy(net.params, g_host_params.data(), sizeof(float) * DAWN_SCHED_PARAMS,
+  DAWN_HIP(hipMemcpy(net.params, params.data(), sizeof(float) * DAWN_SCHED_PARAMS,
                      hipMemcpyHostToDevice));
-  DAWN_HIP(hipMemcpy(net.norm, g_host_norm.data(), sizeof(float) * 16, hipMemcpyHostToDevice));
-  (void)device;
+  DAWN_HIP(hipMemcpy(net.norm, norm.data(), sizeof(float) * 16, hipMemcpyHostToDevice));
   return DAWN_OK;
 }
+
+int install_on(int device, SchedNet& net) {
+  (void)device;
+  return install_net(net, g_host_params, g_host_norm);
+}
+
+// Copies the host parameters and normalisation into the store.
+void set_host_net(const float* params, const float* mean, const float* inv_std,
+                  std::vector<float>& hp, std::vector<float>& hn) {
+  hp.assign(params, params + DAWN_SCHED_PARAMS);
+  hn.assign(16, 0.f);
+  for (int i = 0; i < 8; i++) {
+    hn[i] = mean ? mean[i] : 0.f;
+    hn[8 + i] = inv_std ? inv_std[i] : 1.f;
+  }
+}
 }  // namespace
+
+// The single-source engine's network on `device` (installed there on first
+// use) and its normalisation, or null when none is set.
+const float* sched_single_mlp_device(int device, const float** norm) {
+  std::lock_guard<std::mutex> l(g_sched_lock);
+  *norm = nullptr;
+  if (!g_have_single) return nullptr;
+  SchedNet& net = g_single_nets[device];
+  if (!net.params) {
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(device);
+    const int rc = install_net(net, g_single_params, g_single_norm);
+    (void)hipSetDevice(prev);
+    if (rc != DAWN_OK) return nullptr;
+  }
+  *norm = net.norm;
+  return net.params;
+}
+
+// The same without installing (never allocates): null unless already there.
+const float* sched_single_mlp_if_present(int device, const float** norm) {
+  std::lock_guard<std::mutex> l(g_sched_lock);
+  *norm = nullptr;
+  if (!g_have_single) return nullptr;
+  auto it = g_single_nets.find(device);
+  if (it == g_single_nets.end() || !it->second.params) return nullptr;
+  *norm = it->second.norm;
+  return it->second.params;
+}
 
 const float* sched_mlp_device(int device) {
   std::lock_guard<std::mutex> l(g_sched_lock);
@@ -159,12 +211,7 @@ int dawn_sched_set_mlp(const float* params, const float* feat_mean, const float*
     g_have_net = false;
     return DAWN_OK;
   }
-  g_host_params.assign(params, params + DAWN_SCHED_PARAMS);
-  g_host_norm.assign(16, 0.f);
-  for (int i = 0; i < 8; i++) {
-    g_host_norm[i] = feat_mean ? feat_mean[i] : 0.f;
-    g_host_norm[8 + i] = feat_inv_std ? feat_inv_std[i] : 1.f;
-  }
+  set_host_net(params, feat_mean, feat_inv_std, g_host_params, g_host_norm);
   g_have_net = true;
   // Refresh copies already on devices.
   for (auto& kv : g_nets) {
@@ -172,6 +219,26 @@ int dawn_sched_set_mlp(const float* params, const float* feat_mean, const float*
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(kv.first);
     int rc = install_on(kv.first, kv.second);
+    (void)hipSetDevice(prev);
+    if (rc != DAWN_OK) return rc;
+  }
+  return DAWN_OK;
+}
+
+int dawn_sched_set_mlp_single(const float* params, const float* feat_mean,
+                              const float* feat_inv_std) {
+  std::lock_guard<std::mutex> l(g_sched_lock);
+  if (!params) {
+    g_have_single = false;
+    return DAWN_OK;
+  }
+  set_host_net(params, feat_mean, feat_inv_std, g_single_params, g_single_norm);
+  g_have_single = true;
+  for (auto& kv : g_single_nets) {
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(kv.first);
+    const int rc = install_net(kv.second, g_single_params, g_single_norm);
     (void)hipSetDevice(prev);
     if (rc != DAWN_OK) return rc;
   }
@@ -198,6 +265,34 @@ int dawn_sched_trace(const dawn_graph* gc, const int32_t* sources, int32_t count
   std::vector<float> f((size_t)max_rows * 8), t((size_t)max_rows * DAWN_NUM_MODES);
   std::vector<int32_t> chosen(max_rows);
   DAWN_TRY(run_ms_record(g, sources, count, 1, f.data(), t.data(), chosen.data(), max_rows, rows,
+                         policy));
+  for (int32_t i = 0; i < *rows; i++) {
+    if (modes) modes[i] = chosen[i];
+    if (features) std::memcpy(features + (size_t)i * 8, f.data() + (size_t)i * 8, 8 * sizeof(float));
+    if (times_ms)
+      std::memcpy(times_ms + (size_t)i * DAWN_NUM_MODES, t.data() + (size_t)i * DAWN_NUM_MODES,
+                  DAWN_NUM_MODES * sizeof(float));
+  }
+  return DAWN_OK;
+}
+
+int dawn_sched_record_single(const dawn_graph* gc, const int32_t* sources, int32_t count,
+                             int32_t repeats, float* features, float* times_ms,
+                             int32_t* fixed_modes, int32_t max_rows, int32_t* rows) {
+  dawn_graph* g = const_cast<dawn_graph*>(gc);
+  if (!g || !features || !times_ms || !rows || max_rows < 0) return DAWN_ERR_INVALID_ARG;
+  return run_ss_record(g, sources, count, repeats, features, times_ms, fixed_modes, max_rows,
+                       rows, DAWN_POLICY_FIXED);
+}
+
+int dawn_sched_trace_single(const dawn_graph* gc, const int32_t* sources, int32_t count,
+                            int32_t policy, int32_t* modes, float* features, float* times_ms,
+                            int32_t max_rows, int32_t* rows) {
+  dawn_graph* g = const_cast<dawn_graph*>(gc);
+  if (!g || !rows || max_rows < 0) return DAWN_ERR_INVALID_ARG;
+  std::vector<float> f((size_t)max_rows * 8), t((size_t)max_rows * DAWN_NUM_MODES);
+  std::vector<int32_t> chosen(max_rows);
+  DAWN_TRY(run_ss_record(g, sources, count, 1, f.data(), t.data(), chosen.data(), max_rows, rows,
                          policy));
   for (int32_t i = 0; i < *rows; i++) {
     if (modes) modes[i] = chosen[i];

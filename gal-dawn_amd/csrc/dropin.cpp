@@ -6,12 +6,15 @@
 
 #include <getopt.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../include/dawn_capi.h"
@@ -109,25 +112,30 @@ float multi(DAWN::Graph::Graph_t& graph, std::string& output_path, const std::ve
 
 namespace DAWN {
 
-void Graph::createGraph(std::string& input_path, Graph_t& graph) {
+namespace {
+
+// Reads path with the reader's semantics (as_directed: -1 from the header,
+// 0 both directions, 1 as listed) into graph.csr (plain new[] arrays owned
+// by Graph_t, as in the reference, src/matrix.cpp:32-34).  Returns false
+// after printing the reference's message when the file cannot be used.
+bool read_into(std::string& input_path, DAWN::Graph::Graph_t& graph, bool weighted,
+               int as_directed) {
   dawn_csr c{};
-  const int rc = dawn_read_mtx(input_path.c_str(), graph.weighted ? 1 : 0, &c);
+  const int rc = dawn_read_mtx_as(input_path.c_str(), weighted ? 1 : 0, as_directed, &c);
   if (rc == DAWN_ERR_IO) {
     std::cerr << "Error opening file " << input_path << std::endl;  // graph.cpp:11-14
-    return;
+    return false;
   }
   if (rc == DAWN_ERR_FORMAT) {
     std::cout << "invalid file" << std::endl;  // graph.cpp:45-48
-    return;
+    return false;
   }
-  if (rc != DAWN_OK) die(rc, "dawn_read_mtx");
-  std::cout << "Read Input Graph" << std::endl;
+  if (rc != DAWN_OK) die(rc, "dawn_read_mtx_as");
   graph.rows = c.n;
   graph.cols = c.n;
   graph.nnz = (uint64_t)c.m;
   graph.directed = c.directed != 0;
   graph.source = c.n ? norm_source(graph.source, c.n) : 0;
-  // Graph_t owns plain new[] arrays, as in the reference (matrix.cpp:32-34).
   graph.csr.row_ptr = new int[(size_t)c.n + 1];
   graph.csr.col = new int[(size_t)c.m > 0 ? (size_t)c.m : 1];
   std::memcpy(graph.csr.row_ptr, c.row_ptr, sizeof(int) * ((size_t)c.n + 1));
@@ -137,6 +145,74 @@ void Graph::createGraph(std::string& input_path, Graph_t& graph) {
     for (int64_t k = 0; k < c.m; k++) graph.csr.val[k] = (float)c.weight[k];
   }
   dawn_csr_free(&c);
+  return true;
+}
+
+}  // namespace
+
+void Graph::createGraph(std::string& input_path, Graph_t& graph) {
+  if (read_into(input_path, graph, graph.weighted, -1))
+    std::cout << "Read Input Graph" << std::endl;
+}
+
+// The four readers with their own edge semantics, whatever the header says
+// (src/io.cpp:2-192): undirected ones push both directions of each pair.
+void IO::readGraph(std::string& input_path, Graph::Graph_t& graph) {
+  read_into(input_path, graph, false, 0);
+}
+
+void IO::readGraph_Weighted(std::string& input_path, Graph::Graph_t& graph) {
+  read_into(input_path, graph, true, 0);
+}
+
+void IO::readGraph_Directed(std::string& input_path, Graph::Graph_t& graph) {
+  read_into(input_path, graph, false, 1);
+}
+
+void IO::readGraph_Directed_Weighted(std::string& input_path, Graph::Graph_t& graph) {
+  read_into(input_path, graph, true, 1);
+}
+
+// Row-sorted COO -> CSR on the host, as the reference's helpers
+// (src/matrix.cpp:28-82): rows counted, offsets by a prefix sum, columns
+// (and values) copied in COO order, so the COO must already be sorted by
+// row.  The CSR arrays are new[] arrays owned by the caller.
+void Matrix::coo2Csr(int n, int nnz, Csr_t& csr, Coo_t& coo) {
+  csr.row_ptr = new int[(size_t)n + 1]();
+  csr.col = new int[nnz > 0 ? (size_t)nnz : 1];
+  for (int i = 0; i < nnz; i++) csr.row_ptr[coo.row[i] + 1]++;
+  for (int i = 0; i < n; i++) csr.row_ptr[i + 1] += csr.row_ptr[i];
+  if (nnz > 0) std::memcpy(csr.col, coo.col, sizeof(int) * (size_t)nnz);
+}
+
+void Matrix::coo2Csr_Weighted(int n, int nnz, Csr_t& csr, Coo_t& coo) {
+  coo2Csr(n, nnz, csr, coo);
+  csr.val = new float[nnz > 0 ? (size_t)nnz : 1];
+  if (nnz > 0) std::memcpy(csr.val, coo.val, sizeof(float) * (size_t)nnz);
+}
+
+// Despite the name, the reference's transpose sorts the COO entries by
+// (row, column[, value]) in place (src/matrix.cpp:3-26); the readers of
+// directed graphs use it to order their entries.
+void Matrix::transpose(int nnz, Coo_t& coo) {
+  std::vector<std::pair<int, int>> e((size_t)std::max(nnz, 0));
+  for (int i = 0; i < nnz; i++) e[i] = {coo.row[i], coo.col[i]};
+  std::sort(e.begin(), e.end());
+  for (int i = 0; i < nnz; i++) {
+    coo.row[i] = e[i].first;
+    coo.col[i] = e[i].second;
+  }
+}
+
+void Matrix::transpose_Weighted(int nnz, Coo_t& coo) {
+  std::vector<std::tuple<int, int, float>> e((size_t)std::max(nnz, 0));
+  for (int i = 0; i < nnz; i++) e[i] = std::make_tuple(coo.row[i], coo.col[i], coo.val[i]);
+  std::sort(e.begin(), e.end());
+  for (int i = 0; i < nnz; i++) {
+    coo.row[i] = std::get<0>(e[i]);
+    coo.col[i] = std::get<1>(e[i]);
+    coo.val[i] = std::get<2>(e[i]);
+  }
 }
 
 void IO::readList(std::string& input_path, Graph::Graph_t& graph) {
@@ -164,6 +240,30 @@ void IO::outfile(int n, float* result, int source, std::string& output_path) {
   }
   for (int j = 0; j < n; j++)
     if (source != j && result[j] != 0) std::fprintf(f, "%d %d %g\n", source, j, (double)result[j]);
+  std::fclose(f);
+}
+
+// Source-less forms (src/io.cpp:252-281): "j dist" lines, int values > 0,
+// float values != 0 with six decimals.
+void IO::outfile(int n, int* result, std::string& output_path) {
+  FILE* f = std::fopen(output_path.c_str(), "w");
+  if (!f) {
+    std::cerr << "Error opening file " << output_path << std::endl;
+    return;
+  }
+  for (int j = 0; j < n; j++)
+    if (result[j] > 0) std::fprintf(f, "%d %d\n", j, result[j]);
+  std::fclose(f);
+}
+
+void IO::outfile(int n, float* result, std::string& output_path) {
+  FILE* f = std::fopen(output_path.c_str(), "w");
+  if (!f) {
+    std::cerr << "Error opening file " << output_path << std::endl;
+    return;
+  }
+  for (int j = 0; j < n; j++)
+    if (result[j] != 0) std::fprintf(f, "%d %.6f\n", j, (double)result[j]);
   std::fclose(f);
 }
 
@@ -291,8 +391,22 @@ static void closeness(Graph::Graph_t& graph, int source, bool weighted, double* 
   int32_t s = source;
   int64_t sum = 0;
   dawn_stats st{};
-  const int rc = dawn_closeness(g, &s, 1, exact, &sum, ref, DAWN_POLICY_LEARNED, &st);
-  if (rc != DAWN_OK) die(rc, "dawn_closeness");
+  if (!weighted) {
+    const int rc = dawn_closeness(g, &s, 1, exact, &sum, ref, DAWN_POLICY_LEARNED, &st);
+    if (rc != DAWN_OK) die(rc, "dawn_closeness");
+  } else {
+    // The reference's weighted value (src/algorithm/cpu/cc.cpp:132-133):
+    // the float distance row, FLT_MAX for unreached vertices, accumulated
+    // in order from 0.0f, then (row - 1) / sum in double, as float.  So one
+    // unreached vertex gives about (n-1)/FLT_MAX and two give 0.
+    std::vector<int32_t> row((size_t)graph.rows);
+    const int rc = dawn_mssp(g, &s, 1, row.data(), &sum, DAWN_POLICY_LEARNED, &st);
+    if (rc != DAWN_OK) die(rc, "dawn_mssp");
+    float acc = 0.0f;
+    for (int j = 0; j < graph.rows; j++) acc += row[j] == DAWN_INF ? FLT_MAX : (float)row[j];
+    *ref = (float)((1.0 * graph.rows - 1.0f) / acc);
+    *exact = sum > 0 ? (double)(graph.rows - 1) / (double)sum : 0.0;
+  }
   dawn_graph_destroy(g);
   std::printf("%-21s%3.5d\n", "Node:", graph.rows);  // cc.cu:96-100
   std::printf("%-21s%3.5ld\n", "Edges:", (long)graph.nnz);
@@ -310,7 +424,7 @@ float CC_GPU::run_Weighted(Graph::Graph_t& graph, int source) {
   double exact = 0;
   float ref = 0;
   closeness(graph, source, true, &exact, &ref);
-  return (float)exact;
+  return ref;
 }
 
 double CC_GPU::run_exact(Graph::Graph_t& graph, int source) {

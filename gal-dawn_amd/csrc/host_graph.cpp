@@ -367,7 +367,11 @@ int dawn_csr_from_edges(int32_t n, int64_t count, const int32_t* u,
 }
 
 int dawn_read_mtx(const char* path, int32_t weighted, dawn_csr* out) {
-  if (!path || !out) return DAWN_ERR_INVALID_ARG;
+  return dawn_read_mtx_as(path, weighted, -1, out);
+}
+
+int dawn_read_mtx_as(const char* path, int32_t weighted, int32_t as_directed, dawn_csr* out) {
+  if (!path || !out || as_directed < -1 || as_directed > 1) return DAWN_ERR_INVALID_ARG;
   FILE* f = std::fopen(path, "rb");
   if (!f) return DAWN_ERR_IO;
   std::fseek(f, 0, SEEK_END);
@@ -388,7 +392,8 @@ int dawn_read_mtx(const char* path, int32_t weighted, dawn_csr* out) {
   if (nf < 3 || std::strcmp(fmt, "%%MatrixMarket") != 0 || std::strcmp(mtype, "coordinate") != 0)
     return DAWN_ERR_FORMAT;  // reference prints "invalid file" (src/graph.cpp:45-48)
   const bool pattern = std::strcmp(dtype, "pattern") == 0;
-  const bool directed = std::strcmp(sym, "symmetric") != 0;  // src/graph.cpp:40-44
+  // src/graph.cpp:40-44, unless the caller names the reader (src/io.cpp:2-192).
+  const bool directed = as_directed >= 0 ? as_directed == 1 : std::strcmp(sym, "symmetric") != 0;
   // A pattern file read as weighted uses weight 1 (the reference reads an
   // unset value there, src/io.cpp:107).
   const bool use_w = weighted != 0;

@@ -86,9 +86,10 @@ struct MsBatch {
   int32_t nlanes;
   int32_t lo;                      // first level not yet flushed to dist
   uint64_t lanemask[kMaxWords];    // valid lanes per word
-  int32_t* dist;                   // output rows: lane b at dist + b * stride
+  int32_t* dist;                   // output rows: lane b at dist + row[b] * stride
   int64_t stride;
   int32_t src[kMaxLanes];
+  int32_t row[kMaxLanes];          // output row of each lane
 };
 
 struct MsArgs {
@@ -141,6 +142,7 @@ struct MsWorkspace {
   DevBuf<uint64_t> seen, next, hist, seen_bak;
   DevBuf<uint32_t> fbits, done, done_bak;
   DevBuf<int32_t> q, qb, distbuf;
+  DevBuf<int32_t> iso;  // (row, source) pairs of isolated sources
   int64_t qcap = 0;
   DevBuf<LevelCounters> ring, ring_bak;
   DevBuf<SchedState> st;
@@ -1222,6 +1224,31 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
   }
 }
 
+// Rows of isolated sources: INF except 0 at the source.  rs holds (row,
+// source) pairs; row k goes to dist + rs[2k] * stride.  Four entries per
+// thread, 16-byte streaming stores where the rows are aligned.
+__global__ void __launch_bounds__(kBlock) ms_isolated_rows_kernel(int32_t* dist, int64_t stride,
+                                                                  int32_t n, const int32_t* rs,
+                                                                  int32_t count) {
+  const int64_t per_row = ((int64_t)n + 3) / 4;
+  const int64_t total = per_row * count;
+  const bool vec = (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0);
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int64_t k = t / per_row;
+    const int64_t c = (t - k * per_row) * 4;
+    const int32_t row = rs[2 * k], src = rs[2 * k + 1];
+    int32_t* p = dist + (int64_t)row * stride + c;
+    if (vec && c + 4 <= n) {
+      v4i32 val = {kInf, kInf, kInf, kInf};
+      if (src >= c && src < c + 4) val[src - c] = 0;
+      __builtin_nontemporal_store(val, reinterpret_cast<v4i32*>(p));
+    } else {
+      for (int j = 0; j < 4 && c + j < n; j++) p[j] = c + j == src ? 0 : kInf;
+    }
+  }
+}
+
 // out = a + b on N-bit numbers held as bit slices (slice t: bit t of every
 // lane's number, one 64-bit mask per slice); out has N + 1 slices.
 template <int N>
@@ -1271,10 +1298,12 @@ ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
   const int nl = a.batch->nlanes;
   int32_t* const dist = a.batch->dist;
   const int64_t stride = a.batch->stride;
+  __shared__ int32_t s_row[64 * W];
   for (int i = threadIdx.x; i < 64 * W; i += kBlock) {
     s_sum[i] = 0;
     s_reach[i] = 0;
     s_edges[i] = 0;
+    s_row[i] = i < nl ? a.batch->row[i] : 0;
   }
   __syncthreads();
   // Levels are stored relative to lo in K bit slices: S_k = OR of the
@@ -1298,7 +1327,10 @@ ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
   const bool first = lo == 0;
   const bool vec_ok = dist && (stride % 4 == 0) && (((uintptr_t)dist & 15) == 0) && first;
   const int64_t wstride = (int64_t)(gridDim.x / BPT) * TPB * 256;
-  for (int64_t base = tile0 * 256; base < a.n; base += wstride) {
+  // A word without lanes in the batch (a partly filled last batch) has no
+  // rows and no sums: its wave skips the tiles (no barrier inside the loop).
+  const int64_t tiles_end = rows > 0 ? a.n : 0;
+  for (int64_t base = tile0 * 256; base < tiles_end; base += wstride) {
     const int64_t v0 = base + 4 * lane;
     uint64_t F[4], S[4][KS];
 #pragma unroll
@@ -1402,7 +1434,8 @@ ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
           for (int k = 0; k < KS; k++) Sh[j][k] = (uint32_t)(S[j][k] >> (32 * h));
         }
         const int bmax = min(32, rows - 32 * h);
-        int32_t* const row0 = dist + (int64_t)(64 * wi + 32 * h) * stride + v0;
+        // Lane rb0 + bb of the batch writes output row s_row[rb0 + bb].
+        const int rb0 = 64 * wi + 32 * h;
         // Level of vertex j in row bb relative to lo (slices k >= K are zero).
         auto rel = [&](int j, int bb) {
           uint32_t x = 0;
@@ -1422,11 +1455,12 @@ ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
               const int32_t m = __builtin_amdgcn_sbfe((int32_t)Fh[j], bb, 1);
               val[j] = (m & (lo + rel(j, bb))) | (~m & kInf);
             }
-            __builtin_nontemporal_store(val, reinterpret_cast<v4i32*>(row0 + (int64_t)bb * stride));
+            __builtin_nontemporal_store(
+                val, reinterpret_cast<v4i32*>(dist + (int64_t)s_row[rb0 + bb] * stride + v0));
           }
         } else {
           for (int bb = 0; bb < bmax; bb++) {
-            int32_t* row = row0 + (int64_t)bb * stride;
+            int32_t* row = dist + (int64_t)s_row[rb0 + bb] * stride + v0;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
               const bool in = __builtin_amdgcn_ubfe(Fh[j], bb, 1) != 0;
@@ -1637,8 +1671,18 @@ void stage_batch(MsWorkspace* ws, const int32_t* sources, int32_t lanes, const M
     int64_t s = sources[i] % (int64_t)ws->n;
     if (s < 0) s += ws->n;
     b.src[i] = view ? view->h_perm[s] : (int32_t)s;
+    b.row[i] = i;  // lane i -> row i of the output (callers with a row map override)
   }
 }
+
+// Sources without out-edges are not traversed (the reference skips them,
+// src/algorithm/cpu/mssp.cpp:16-18): their row is INF except 0 at the
+// source.  Only the other sources take lanes, so every lane of a batch
+// traverses.  Pairs (output row, source vertex) of the isolated sources.
+struct IsoRows {
+  const int32_t* pairs = nullptr;
+  int32_t count = 0;
+};
 
 template <int W>
 void launch_pull(const MsArgs& a, unsigned grid, hipStream_t s) {
@@ -1667,9 +1711,9 @@ void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t 
 //   Processes batches first, first + step, ... on workspace ws and stream s.
 template <int W>
 int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t* sources,
-                   int32_t S, int32_t first, int32_t step, int32_t* dist, int64_t* sums,
-                   int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats,
-                   bool rows) {
+                   const int32_t* rowidx, int32_t S, int32_t first, int32_t step,
+                   const IsoRows* iso, int32_t* dist, int64_t* sums, int64_t* reached,
+                   int64_t* edges, int32_t policy, dawn_stats* stats, bool rows) {
   constexpr int kB = 64 * W;
   const bool dist_dev = dist && is_device_pointer(dist);
   // Rows of a host output, or (rows) of a call without output, go through
@@ -1718,12 +1762,37 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
   std::vector<unsigned long long> h_ls(3 * kMaxLanes);
   std::vector<int32_t> h_modes(kTraceMax);
   float total_ms = 0.f;
+  // Output row of entry k of sources.
+  auto row_of = [&](int64_t k) -> int64_t { return rowidx ? rowidx[k] : k; };
+  // Rows of the isolated sources (timed with the batches): into the
+  // caller's device rows, or the slot's row buffer when rows are computed
+  // without an output (row k mod 64·W there).
+  if (iso && iso->count > 0 && (dist_dev || buf_rows)) {
+    std::vector<int32_t> pairs(iso->pairs, iso->pairs + 2 * (size_t)iso->count);
+    if (!dist_dev)
+      for (int32_t k = 0; k < iso->count; k++) pairs[2 * k] = k % kB;
+    DAWN_TRY(ws->iso.alloc(pairs.size()));
+    DAWN_HIP(hipMemcpyAsync(ws->iso.p, pairs.data(), sizeof(int32_t) * pairs.size(),
+                            hipMemcpyHostToDevice, s));
+    DAWN_HIP(hipEventRecord(ws->ev0, s));
+    const int64_t vecs = ((int64_t)g->n + 3) / 4 * iso->count;
+    ms_isolated_rows_kernel<<<grid_for(vecs, kBlock, 65536u), kBlock, 0, s>>>(
+        dist_dev ? dist : ws->distbuf.p, g->n, g->n, ws->iso.p, iso->count);
+    DAWN_HIP(hipGetLastError());
+    DAWN_HIP(hipEventRecord(ws->ev1, s));
+    DAWN_HIP(hipEventSynchronize(ws->ev1));
+    float ms = 0.f;
+    DAWN_HIP(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+    total_ms += ms;
+  }
   for (int32_t b0 = first * kB; b0 < S; b0 += step * kB) {
     const int32_t lanes = std::min<int32_t>(kB, S - b0);
-    int32_t* out_base = dist_dev ? dist + (int64_t)b0 * g->n : buf_rows ? ws->distbuf.p : nullptr;
+    int32_t* out_base = dist_dev ? dist : buf_rows ? ws->distbuf.p : nullptr;
     hipGraphExec_t exec;
     DAWN_TRY(get_graph(want_edges, &exec));
     stage_batch(ws, sources + b0, lanes, g->view);
+    if (dist_dev)
+      for (int i = 0; i < lanes; i++) ws->h_batch->row[i] = (int32_t)row_of(b0 + i);
     ws->h_batch->dist = out_base;
     ws->h_batch->stride = g->n;
     DAWN_HIP(hipMemcpyAsync(ws->batch.p, ws->h_batch, sizeof(MsBatch), hipMemcpyHostToDevice, s));
@@ -1769,23 +1838,31 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
     DAWN_HIP(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
     total_ms += ms;
     if (dist && !dist_dev) {
-      DAWN_HIP(hipMemcpyAsync(dist + (int64_t)b0 * g->n, ws->distbuf.p,
-                              sizeof(int32_t) * (size_t)lanes * g->n, hipMemcpyDeviceToHost, s));
+      // Lane rows of the row buffer to their rows of the host output, runs
+      // of consecutive rows in one copy.
+      for (int i = 0; i < lanes;) {
+        const int64_t r0 = row_of(b0 + i);
+        int j = i + 1;
+        while (j < lanes && row_of(b0 + j) == r0 + (j - i)) j++;
+        DAWN_HIP(hipMemcpyAsync(dist + r0 * g->n, ws->distbuf.p + (int64_t)i * g->n,
+                                sizeof(int32_t) * (size_t)(j - i) * g->n, hipMemcpyDeviceToHost,
+                                s));
+        i = j;
+      }
       DAWN_HIP(hipStreamSynchronize(s));
     }
     for (int i = 0; i < lanes; i++) {
-      if (sums) sums[b0 + i] = (int64_t)h_ls[i];
-      if (reached) reached[b0 + i] = (int64_t)h_ls[kMaxLanes + i];
-      if (edges) edges[b0 + i] = (int64_t)h_ls[2 * kMaxLanes + i];
+      const int64_t r = row_of(b0 + i);
+      if (sums) sums[r] = (int64_t)h_ls[i];
+      if (reached) reached[r] = (int64_t)h_ls[kMaxLanes + i];
+      if (edges) edges[r] = (int64_t)h_ls[2 * kMaxLanes + i];
     }
     if (stats) {
       stats->batches += 1;
+      stats->frontier_edges += (int64_t)ws->h_state->work_fe;
       for (int i = 0; i < lanes; i++) {
         stats->vertices_reached += (int64_t)h_ls[kMaxLanes + i];
         stats->edges_reached += (int64_t)h_ls[2 * kMaxLanes + i];
-        const int32_t sv = ws->h_batch->src[i];
-        const std::vector<int32_t>& rp = g->view ? g->view->h_row_ptr : g->h_row_ptr;
-        if (rp[sv] == rp[sv + 1]) stats->skipped_sources += 1;
       }
       const int levels = ws->h_state->levels;
       for (int st = 1; st <= levels && st < kTraceMax; st++) {
@@ -1806,8 +1883,9 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
 // second workspace cannot be allocated, one stream runs them all.  Then
 // stats->device_ms is the wall time of the whole set of batches.
 template <int W>
-int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-                int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows) {
+int run_msbfs_w(dawn_graph* g, const int32_t* sources, const int32_t* rowidx, int32_t S,
+                const IsoRows* iso, int32_t* dist, int64_t* sums, int64_t* reached,
+                int64_t* edges, int32_t policy, dawn_stats* stats, bool rows) {
   constexpr int kB = 64 * W;
   DAWN_TRY(ensure_ms(g, g->ms, W));
   DAWN_TRY(ms_view_ensure(g));
@@ -1831,8 +1909,8 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     }
   }
   if (!two)
-    return run_msbfs_slot<W>(g, g->ms, g->stream, sources, S, 0, 1, dist, sums, reached, edges,
-                             policy, stats, rows);
+    return run_msbfs_slot<W>(g, g->ms, g->stream, sources, rowidx, S, 0, 1, iso, dist, sums,
+                             reached, edges, policy, stats, rows);
   dawn_stats st1{};
   int rc1 = DAWN_OK;
   const auto t0 = std::chrono::steady_clock::now();
@@ -1841,12 +1919,14 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
       rc1 = DAWN_ERR_HIP;
       return;
     }
-    rc1 = run_msbfs_slot<W>(g, g->ms2, g->stream2, sources, S, 1, 2, dist, sums, reached, edges,
-                            policy, stats ? &st1 : nullptr, rows);
+    // The second stream runs the fewer batches: it also writes the rows of
+    // the isolated sources.
+    rc1 = run_msbfs_slot<W>(g, g->ms2, g->stream2, sources, rowidx, S, 1, 2, iso, dist, sums,
+                            reached, edges, policy, stats ? &st1 : nullptr, rows);
   });
   dawn_stats st0{};
-  const int rc0 = run_msbfs_slot<W>(g, g->ms, g->stream, sources, S, 0, 2, dist, sums, reached,
-                                    edges, policy, stats ? &st0 : nullptr, rows);
+  const int rc0 = run_msbfs_slot<W>(g, g->ms, g->stream, sources, rowidx, S, 0, 2, nullptr, dist,
+                                    sums, reached, edges, policy, stats ? &st0 : nullptr, rows);
   second.join();
   const double wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1860,6 +1940,7 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist,
     stats->vertices_reached += st0.vertices_reached + st1.vertices_reached;
     stats->batches += st0.batches + st1.batches;
     stats->skipped_sources += st0.skipped_sources + st1.skipped_sources;
+    stats->frontier_edges += st0.frontier_edges + st1.frontier_edges;
   }
   return DAWN_OK;
 }
@@ -2026,29 +2107,80 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
 
 }  // namespace
 
+// Splits a source list into the sources that traverse (out-degree > 0)
+// with their positions in the list, and (position, vertex) pairs of the
+// isolated ones.
+void split_live(const dawn_graph* g, const int32_t* sources, int32_t S, std::vector<int32_t>& live,
+                std::vector<int32_t>& live_row, std::vector<int32_t>& iso) {
+  live.clear();
+  live_row.clear();
+  iso.clear();
+  for (int32_t i = 0; i < S; i++) {
+    int64_t s = sources[i] % (int64_t)g->n;
+    if (s < 0) s += g->n;
+    if (g->h_row_ptr[s] == g->h_row_ptr[s + 1]) {
+      iso.push_back(i);
+      iso.push_back((int32_t)s);
+    } else {
+      live.push_back(sources[i]);
+      live_row.push_back(i);
+    }
+  }
+}
+
 int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
               int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows) {
   if (!g || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> guard(g->lock);
   DAWN_HIP(hipSetDevice(g->device));
+  // Isolated sources take no lane (as the reference skips them,
+  // src/algorithm/cpu/mssp.cpp:16-18): sum 0, reached 1 (the source), no
+  // edges; their rows are written by the engine when rows are asked for.
+  std::vector<int32_t> live, live_row, iso;
+  split_live(g, sources, S, live, live_row, iso);
+  const int32_t L = (int32_t)live.size();
+  const IsoRows isr{iso.data(), (int32_t)(iso.size() / 2)};
+  const bool dist_dev = dist && is_device_pointer(dist);
   // Workspaces are sized by the batch width; if the device cannot hold them
   // (they are allocated before any batch runs), narrower batches are tried.
-  for (int w = words_for(g, S);; w >>= 1) {
+  for (int w = words_for(g, L);; w >>= 1) {
     int rc;
+    const int32_t* ls = live.data();
+    const int32_t* lr = live_row.data();
     switch (w) {
-      case 1: rc = run_msbfs_w<1>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
-      case 2: rc = run_msbfs_w<2>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
-      case 4: rc = run_msbfs_w<4>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
-      case 8: rc = run_msbfs_w<8>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
-      default: rc = run_msbfs_w<16>(g, sources, S, dist, sums, reached, edges, policy, stats, rows); break;
+      case 1: rc = run_msbfs_w<1>(g, ls, lr, L, &isr, dist, sums, reached, edges, policy, stats, rows); break;
+      case 2: rc = run_msbfs_w<2>(g, ls, lr, L, &isr, dist, sums, reached, edges, policy, stats, rows); break;
+      case 4: rc = run_msbfs_w<4>(g, ls, lr, L, &isr, dist, sums, reached, edges, policy, stats, rows); break;
+      case 8: rc = run_msbfs_w<8>(g, ls, lr, L, &isr, dist, sums, reached, edges, policy, stats, rows); break;
+      default: rc = run_msbfs_w<16>(g, ls, lr, L, &isr, dist, sums, reached, edges, policy, stats, rows); break;
     }
-    if (rc != DAWN_ERR_OUT_OF_MEMORY || w == 1) return rc;
-    ms_workspace_free(g->ms);
-    g->ms = nullptr;
-    ms_workspace_free(g->ms2);
-    g->ms2 = nullptr;
-    (void)hipGetLastError();
+    if (rc == DAWN_ERR_OUT_OF_MEMORY && w > 1) {
+      ms_workspace_free(g->ms);
+      g->ms = nullptr;
+      ms_workspace_free(g->ms2);
+      g->ms2 = nullptr;
+      (void)hipGetLastError();
+      continue;
+    }
+    if (rc != DAWN_OK) return rc;
+    break;
   }
+  for (int32_t k = 0; k < isr.count; k++) {
+    const int32_t i = iso[2 * k], s = iso[2 * k + 1];
+    if (sums) sums[i] = 0;
+    if (reached) reached[i] = 1;
+    if (edges) edges[i] = 0;
+    if (dist && !dist_dev) {
+      int32_t* row = dist + (int64_t)i * g->n;
+      std::fill(row, row + g->n, (int32_t)kInf);
+      row[s] = 0;
+    }
+  }
+  if (stats) {
+    stats->skipped_sources += isr.count;
+    stats->vertices_reached += isr.count;
+  }
+  return DAWN_OK;
 }
 
 int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
@@ -2057,6 +2189,15 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
   if (!g || S <= 0 || !sources || repeats < 1) return DAWN_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> guard(g->lock);
   DAWN_HIP(hipSetDevice(g->device));
+  // The batches of a traversal: isolated sources take no lane.
+  std::vector<int32_t> live, live_row, iso;
+  split_live(g, sources, S, live, live_row, iso);
+  sources = live.data();
+  S = (int32_t)live.size();
+  if (S == 0) {
+    if (rows_out) *rows_out = 0;
+    return DAWN_OK;
+  }
   switch (words_for(g, S)) {
     case 1:
       return run_ms_record_w<1>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
@@ -2081,6 +2222,19 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   if (!g || S <= 0 || !sources || !kernel_ms || !launches) return DAWN_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> guard(g->lock);
   DAWN_HIP(hipSetDevice(g->device));
+  // The batches of a traversal: isolated sources take no lane (their rows
+  // are a plain fill, not profiled here).
+  std::vector<int32_t> live, live_row, iso;
+  split_live(g, sources, S, live, live_row, iso);
+  const int32_t S_all = S;
+  sources = live.data();
+  S = (int32_t)live.size();
+  if (levels_out)
+    for (int32_t i = 0; i < (S_all + 63) / 64; i++) levels_out[i] = -1;
+  if (S == 0) {
+    for (int i = 0; i < 8; i++) kernel_ms[i] = 0, launches[i] = 0;
+    return DAWN_OK;
+  }
   switch (words_for(g, S)) {
     case 1: return run_ms_profile_w<1>(g, sources, S, policy, kernel_ms, launches, levels_out);
     case 2: return run_ms_profile_w<2>(g, sources, S, policy, kernel_ms, launches, levels_out);

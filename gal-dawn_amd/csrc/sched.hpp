@@ -42,6 +42,8 @@ struct SchedState {
   int32_t force_mode;// >= 0: every step uses this mode
   int32_t source;    // single-source engines: the traversal's source (read by
                      // captured step kernels; never a baked launch argument)
+  unsigned long long work_fv;  // frontier vertices expanded, summed over the steps
+  unsigned long long work_fe;  // their out-edges (the relaxations of a push schedule)
 };
 
 // Constant per traversal.
@@ -187,6 +189,8 @@ __device__ inline void sched_decide(SchedState* sp, const LevelCounters* ring,
         if (!p.allow_pull && mode == DAWN_MODE_PULL) mode = DAWN_MODE_BITMAP;
         s.step += 1;
         s.mode = mode;
+        s.work_fv += c.fv;  // the level this step expands
+        s.work_fe += c.fe;
       }
     }
     *sp = s;
@@ -212,6 +216,11 @@ __device__ inline void sched_decide(SchedState* sp, const LevelCounters* ring,
   }
 }
 
+// The single-source engine's network on `device` (installed there on first
+// use) and its normalisation, or null when none is set; the variant that
+// never allocates returns null unless it is already there.
+const float* sched_single_mlp_device(int device, const float** norm);
+const float* sched_single_mlp_if_present(int device, const float** norm);
 // Host-side view of the installed scheduler network (sched.cpp).
 const float* sched_mlp_device(int device);
 const float* sched_norm_device(int device);

@@ -605,8 +605,9 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
   p.allow_pull = 1;
   p.avg_degree = g->n ? (float)((double)g->m / g->n) : 0.f;
   p.max_degree = (float)g->max_out_degree;
-  p.mlp = sched_mlp_device(g->device);
-  p.mlp_norm = sched_norm_device(g->device);
+  // The single-source network (trained on single-source steps); without
+  // one the learned policy takes the fixed rule.  Min-plus: the fixed rule.
+  p.mlp = kind == 0 ? sched_single_mlp_device(g->device, &p.mlp_norm) : nullptr;
   if (kind == 1 && policy == DAWN_POLICY_LEARNED) p.policy = DAWN_POLICY_FIXED;
   SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
 
@@ -689,6 +690,7 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
     stats->vertices_reached += (int64_t)rr[0];
     stats->edges_reached += (int64_t)rr[1];
     stats->batches += 1;
+    stats->frontier_edges += (int64_t)ws->h_state->work_fe;
     // Steps first_step..levels expanded a level; the final step found none.
     for (int st = first_step; st <= levels && st < kTraceMax; st++) {
       stats->steps += 1;
@@ -719,6 +721,130 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     DAWN_HIP(hipStreamSynchronize(s));
   }
+  return DAWN_OK;
+}
+
+// ---- per-step mode timings (scheduler training data) ------------------------
+//
+// Every mode discovers the same levels (the frontier of level l is the set
+// of vertices at hop distance l, whatever the modes of the steps before), so
+// a step starts from the same state in every run: one BOVM traversal per
+// mode with that mode forced, each step launched on its own with HIP events
+// around the expand kernel, gives the time of every mode at every step
+// (minimum over `repeats` traversals); one traversal under `policy` gives
+// the step features and the policy's choices.
+
+namespace {
+
+// One traversal from a.source, step by step.  force < 0: the policy's
+// choices, else that mode at every step.  times: expand-kernel time of each
+// step (steps 1..levels).
+int ss_stepwise(dawn_graph* g, SsWorkspace* ws, const SsArgs& a, const SchedParams& p,
+                SchedTrace tr, int force, unsigned grid, std::vector<float>* times,
+                int* levels) {
+  hipStream_t s = g->stream;
+  ss_fill_kernel<<<grid_for(std::max<int64_t>(3 * ws->words, g->n)), kBlock, 0, s>>>(a, 0);
+  bfs_seed_kernel<<<1, 64, 0, s>>>(a, p.policy);
+  DAWN_HIP(hipGetLastError());
+  if (force >= 0) {
+    const int32_t fm = force;
+    DAWN_HIP(hipMemcpyAsync(&ws->st.p->force_mode, &fm, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
+  times->clear();
+  for (int64_t k = 0; k <= (int64_t)g->n + 1; k++) {
+    ss_decide_kernel<<<1, 64, 0, s>>>(a, p, tr);
+    DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(SchedState), hipMemcpyDeviceToHost, s));
+    DAWN_HIP(hipStreamSynchronize(s));
+    if (ws->h_state->done) {
+      *levels = ws->h_state->levels;
+      return DAWN_OK;
+    }
+    DAWN_HIP(hipEventRecord(ws->ev0, s));
+    bfs_expand_kernel<<<grid, kBlock, 0, s>>>(a);
+    DAWN_HIP(hipGetLastError());
+    DAWN_HIP(hipEventRecord(ws->ev1, s));
+    DAWN_HIP(hipEventSynchronize(ws->ev1));
+    float ms = 0.f;
+    DAWN_HIP(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+    times->push_back(ms);
+  }
+  return DAWN_ERR_HIP;  // step bound exceeded
+}
+
+}  // namespace
+
+int run_ss_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
+                  float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
+                  int32_t* rows_out, int32_t policy) {
+  if (!g || S < 0 || (S > 0 && !sources) || repeats < 1 || !rows_out || max_rows < 0)
+    return DAWN_ERR_INVALID_ARG;
+  if (g->weighted) return DAWN_ERR_UNSUPPORTED;
+  *rows_out = 0;
+  std::lock_guard<std::mutex> guard(g->lock);
+  DAWN_HIP(hipSetDevice(g->device));
+  DAWN_TRY(ensure_ss(g));
+  SsWorkspace* ws = g->ss;
+  const unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock, 4096);
+  int32_t rows = 0;
+  std::vector<float> t, feat;
+  std::vector<int32_t> modes;
+  for (int32_t i = 0; i < S && rows < max_rows; i++) {
+    int64_t src = sources[i] % (int64_t)g->n;
+    if (src < 0) src += g->n;
+    if (g->h_row_ptr[src] == g->h_row_ptr[src + 1]) continue;  // isolated: no traversal
+    SsArgs a{};
+    a.n = g->n;
+    a.source = (int32_t)src;
+    a.words = ws->words;
+    a.row_ptr = g->row_ptr;
+    a.col = g->col;
+    a.w = g->weight;
+    a.in_row_ptr = g->in_row_ptr;
+    a.in_col = g->in_col;
+    a.in_w = g->in_weight;
+    a.dist = ws->dist.p;
+    a.q = ws->q.p;
+    a.bm = ws->bm.p;
+    a.ring = ws->ring.p;
+    a.st = ws->st.p;
+    SchedParams p{};
+    p.n = g->n;
+    p.m = g->m;
+    p.policy = policy;
+    p.alpha = 15;
+    p.beta = 18;
+    p.batched = 0;
+    p.allow_pull = 1;
+    p.avg_degree = g->n ? (float)((double)g->m / g->n) : 0.f;
+    p.max_degree = (float)g->max_out_degree;
+    p.mlp = sched_single_mlp_device(g->device, &p.mlp_norm);
+    SchedTrace tr{ws->trace_feat.p, ws->trace_mode.p};
+    // The policy's run: features and choices of steps 1..levels.
+    int levels = 0;
+    DAWN_TRY(ss_stepwise(g, ws, a, p, tr, -1, grid, &t, &levels));
+    const int nrec = std::min(levels + 1, kTraceMax);
+    modes.assign(nrec, 0);
+    feat.assign((size_t)nrec * 8, 0.f);
+    DAWN_HIP(hipMemcpy(modes.data(), ws->trace_mode.p, sizeof(int32_t) * nrec, hipMemcpyDeviceToHost));
+    DAWN_HIP(hipMemcpy(feat.data(), ws->trace_feat.p, sizeof(float) * 8 * nrec, hipMemcpyDeviceToHost));
+    // Every mode forced, minimum over the repeats.
+    std::vector<float> best((size_t)levels * DAWN_NUM_MODES, 1e30f);
+    for (int m = 0; m < DAWN_NUM_MODES; m++)
+      for (int r = 0; r < repeats; r++) {
+        int lv = 0;
+        DAWN_TRY(ss_stepwise(g, ws, a, p, tr, m, grid, &t, &lv));
+        if (lv != levels || (int)t.size() != levels) return DAWN_ERR_HIP;  // modes must agree
+        for (int k = 0; k < levels; k++)
+          best[(size_t)k * DAWN_NUM_MODES + m] = std::min(best[(size_t)k * DAWN_NUM_MODES + m], t[k]);
+      }
+    for (int k = 1; k <= levels && k < nrec && rows < max_rows; k++, rows++) {
+      for (int j = 0; j < 8; j++) features[(size_t)rows * 8 + j] = feat[(size_t)k * 8 + j];
+      for (int m = 0; m < DAWN_NUM_MODES; m++)
+        times_ms[(size_t)rows * DAWN_NUM_MODES + m] = best[(size_t)(k - 1) * DAWN_NUM_MODES + m];
+      if (chosen) chosen[rows] = modes[k];
+    }
+  }
+  *rows_out = rows;
   return DAWN_OK;
 }
 

@@ -3,12 +3,18 @@
 // src/algorithm/gpu/bfs.cu:67-68).
 //
 // Knobs (dawn::TuneKnobs): grid cap of the batched dense kernels, the degree
-// from which edge lists are cut into wave chunks, the chunk length, and the
-// single-source steps per captured hipGraph.  dawn_tune sweeps them on a
-// graph (coordinate descent, HIP-event device time of a sample traversal,
-// best of two runs), applies the winner to the graph and caches it for the
-// graph's class (log2 n, log2 average degree, directed, weighted); graphs
-// created later in that class start from the cached knobs.  The cache
+// from which edge lists are cut into wave chunks, the chunk length, the
+// single-source steps per captured hipGraph, the light-pull variant, the
+// batch width and the hub-first order.  dawn_tune sweeps them on a graph
+// (coordinate descent from the graph's current knobs), applies the winner
+// to the graph and caches it for the graph's class (log2 n, log2 average
+// degree, directed, weighted); graphs created later in that class start
+// from the cached knobs.  The objective is the time of the workload the
+// engine serves: unweighted graphs, a multi-source pass over 4096 sources
+// spread uniformly over the vertex range (isolated ones included, as random
+// sources are), in batches of the width being tried with two batches in
+// flight on two streams -- the wall time of the whole pass, best of two
+// runs after a warm-up; weighted graphs, single-source traversals.  The cache
 // persists as text: one line "class dense_blocks heavy_degree pull_chunk
 // ss_chunk pull_variant lane_words hub_order".
 #include <hip/hip_runtime.h>
@@ -46,7 +52,7 @@ std::string class_key(const dawn_graph* g) {
   return buf;
 }
 
-// Device time of a sample traversal with the graph's current knobs.
+// Time of the sample workload with the graph's current knobs.
 int sample_ms(dawn_graph* g, const std::vector<int32_t>& src, double* ms) {
   double best = 1e300;
   for (int rep = 0; rep < 3; rep++) {  // the first run also warms up
@@ -107,11 +113,24 @@ extern "C" {
 int dawn_tune(const dawn_graph* gc, int32_t verbose) {
   dawn_graph* g = const_cast<dawn_graph*>(gc);
   if (!g || g->n <= 0) return DAWN_ERR_INVALID_ARG;
-  // Sample sources: the highest-degree vertex and spread vertices with edges.
+  // Sample sources.  Weighted: two spread vertices with edges (single-source
+  // traversals).  Unweighted: 4096 vertices spread evenly over the range,
+  // isolated ones included, as uniform random sources are.
   std::vector<int32_t> src;
-  const int want = g->weighted ? 2 : 1024;
-  for (int32_t v = 0; v < g->n && (int)src.size() < want; v += std::max(1, g->n / (4 * want)))
-    if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
+  if (g->weighted) {
+    for (int32_t v = 0; v < g->n && (int)src.size() < 2; v += std::max(1, g->n / 8))
+      if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
+  } else {
+    const int64_t want = 4096;
+    for (int64_t i = 0; i < want; i++) src.push_back((int32_t)(i * g->n / want));
+    bool any = false;
+    for (int32_t v : src) any = any || g->h_row_ptr[v + 1] > g->h_row_ptr[v];
+    for (int32_t v = 0; v < g->n && !any; v++)  // a vertex with edges, so that something runs
+      if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) {
+        src.push_back(v);
+        any = true;
+      }
+  }
   if (src.empty()) return DAWN_ERR_INVALID_ARG;
 
   TuneKnobs best = g->knobs;
@@ -147,8 +166,8 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       DAWN_TRY(consider(k));
     }
   } else {
-    // The same 1024 sample sources at every width (1 to 16 batches); the
-    // width first, since the best vertex order and launch shapes depend on it.
+    // The same sample sources at every width; the width first, since the
+    // best vertex order and launch shapes depend on it.
     for (int lw : {1, 2, 4, 8, 16}) {
       TuneKnobs k = best;
       k.lane_words = lw;
@@ -169,7 +188,8 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
       k.dense_blocks = b;
       DAWN_TRY(consider(k));
     }
-    const int pairs[][2] = {{16, 256}, {32, 128}, {32, 256}, {32, 512}, {64, 256}, {64, 1024}};
+    const int pairs[][2] = {{16, 256}, {32, 128}, {32, 256}, {32, 512}, {32, 1024},
+                            {48, 512}, {48, 1024}, {64, 256}, {64, 1024}};
     const TuneKnobs base = best;
     for (auto& p : pairs) {
       TuneKnobs k = base;

@@ -16,7 +16,13 @@ Training graphs (seeds 10-40): RMAT scales 16-23 (ef 16), Erdos-Renyi, and
 grids with 1% long-range edges.  The held-out RMAT scale 24 graph of the
 benchmark is not used here.
 
-  python gal-dawn_amd/train_sched.py [--quick]
+--single trains the single-source engine's network instead (dawn_bfs; its
+own parameter set, dawn_sched_set_mlp_single): the steps of single-source
+traversals from `--sources` sampled vertices of each graph, every mode
+timed at every step (dawn_sched_record_single), saved to
+sched_mlp_single.json and sched_data_single.npz.
+
+  python gal-dawn_amd/train_sched.py [--quick] [--single]
 """
 import argparse
 import json
@@ -56,14 +62,19 @@ def make(kind, size, seed):
     return dawn.HostGraph.grid(size, 0.01, seed)
 
 
-def record(quick: bool, repeats: int, batches: int):
+def record(quick: bool, repeats: int, batches: int, single: bool = False, sources: int = 16):
     X, Y, F, meta = [], [], [], []
     for kind, size, seed in training_graphs(quick):
         h = make(kind, size, seed)
         g = dawn.Graph(h)
-        src = dawn.gen_sources(h.n, 64 * g.knobs()["lane_words"] * batches, seed)
+        if single:
+            src = dawn.gen_sources(h.n, 8 * sources, seed)
+            src = src[h.degrees()[src] > 0][:sources]  # isolated sources have no steps
+        else:
+            src = dawn.gen_sources(h.n, 64 * g.knobs()["lane_words"] * batches, seed)
         t0 = time.time()
-        feat, times, fixed = g.sched_record(src, repeats=repeats)
+        feat, times, fixed = g.sched_record(src, repeats=repeats, single=single,
+                                            max_rows=16384)
         g.close()
         X.append(feat)
         Y.append(times)
@@ -86,17 +97,22 @@ def main():
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--out", default=os.path.join(HERE, "sched_mlp.json"))
+    ap.add_argument("--out", default=None)
     ap.add_argument("--data", default=None, help="reuse a recorded sched_data.npz")
+    ap.add_argument("--single", action="store_true", help="the single-source engine's network")
+    ap.add_argument("--sources", type=int, default=16, help="--single: sources per graph")
     args = ap.parse_args()
+    if args.out is None:
+        args.out = os.path.join(HERE, "sched_mlp_single.json" if args.single else "sched_mlp.json")
 
     if args.data:
         d = np.load(args.data, allow_pickle=False)
         X, T, FX = d["features"], d["times_ms"], d["fixed_modes"]
         meta = json.loads(str(d["meta"]))
     else:
-        X, T, FX, meta = record(args.quick, args.repeats, args.batches)
-        np.savez(os.path.join(os.path.dirname(os.path.abspath(args.out)), "sched_data.npz"),
+        X, T, FX, meta = record(args.quick, args.repeats, args.batches, args.single, args.sources)
+        np.savez(os.path.join(os.path.dirname(os.path.abspath(args.out)),
+                              "sched_data_single.npz" if args.single else "sched_data.npz"),
                  features=X, times_ms=T, fixed_modes=FX,
                  meta=json.dumps(meta))
     Y = np.log2(np.maximum(T, 1e-4) * 1e3).astype(np.float32)  # log2 microseconds

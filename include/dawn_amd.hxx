@@ -5,7 +5,9 @@
 // include and the library:
 //   DAWN::Graph::Graph_t / createGraph     include/dawn/graph.hxx:14-33
 //   DAWN::Matrix::Csr_t / Coo_t            include/dawn/matrix.hxx:7-19
-//   DAWN::IO::readList / outfile           include/dawn/io.hxx:7-24
+//   DAWN::Matrix::coo2Csr* / transpose*    include/dawn/matrix.hxx:21-30
+//   DAWN::IO::readGraph* / readList /
+//             outfile (all four)           include/dawn/io.hxx:7-24
 //   DAWN::IO::parameters                   include/dawn/parameters.hxx:8-17
 //   DAWN::BFS_GPU::run                     include/dawn/algorithm/gpu/bfs.cuh:12
 //   DAWN::SSSP_GPU::run                    include/dawn/algorithm/gpu/sssp.cuh:20
@@ -18,9 +20,12 @@
 // Behaviour kept: run() returns seconds; an isolated source prints the
 // reference's message and exits with status 0 (single-source entry points)
 // or is skipped (MSSP/APSP); graph.print writes "src dst dist" lines.
-// Differences: weights must be integral (int32 min-plus); CC_GPU::run
-// returns the reference's value (row-1)/int32-sum with integer division,
-// CC_GPU::run_exact the engine's (n-1)/int64-sum in double.
+// Differences: weights must be integral (int32 min-plus).  CC_GPU::run
+// returns the reference's value (row-1)/int32-sum with integer division
+// (src/algorithm/cpu/cc.cpp:77-78), CC_GPU::run_Weighted the reference's
+// float value (the distance row with FLT_MAX for unreached vertices summed
+// in float, cc.cpp:132-133), CC_GPU::run_exact the engine's (n-1)/int64-sum
+// over reached vertices in double.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -39,6 +44,12 @@ struct Coo_t {
   int* col = nullptr;
   float* val = nullptr;
 };
+// Row-sorted COO -> CSR (new[] arrays), src/matrix.cpp:28-82.
+void coo2Csr(int n, int nnz, Csr_t& csr, Coo_t& coo);
+void coo2Csr_Weighted(int n, int nnz, Csr_t& csr, Coo_t& coo);
+// In-place sort of the COO by (row, col[, val]), src/matrix.cpp:3-26.
+void transpose(int nnz, Coo_t& coo);
+void transpose_Weighted(int nnz, Coo_t& coo);
 }  // namespace Matrix
 
 namespace Graph {
@@ -67,9 +78,20 @@ void createGraph(std::string& input_path, Graph_t& graph);
 }  // namespace Graph
 
 namespace IO {
+// The MatrixMarket readers with their own edge semantics (src/io.cpp:2-192):
+// the undirected ones push both directions of each listed pair, the
+// directed ones keep them as listed; self-loops dropped, 1-based ids,
+// duplicates kept, rows sorted.  graph.rows/cols/nnz and graph.csr are set.
+void readGraph(std::string& input_path, DAWN::Graph::Graph_t& graph);
+void readGraph_Weighted(std::string& input_path, DAWN::Graph::Graph_t& graph);
+void readGraph_Directed(std::string& input_path, DAWN::Graph::Graph_t& graph);
+void readGraph_Directed_Weighted(std::string& input_path, DAWN::Graph::Graph_t& graph);
 void readList(std::string& input_path, DAWN::Graph::Graph_t& graph);
 void outfile(int n, int* result, int source, std::string& output_path);
 void outfile(int n, float* result, int source, std::string& output_path);
+// "j dist" lines without the source column (src/io.cpp:252-281).
+void outfile(int n, int* result, std::string& output_path);
+void outfile(int n, float* result, std::string& output_path);
 
 struct parameters_t {
   std::string input_path;

@@ -81,6 +81,10 @@ typedef struct dawn_stats {
   int64_t batches;
   int32_t skipped_sources;   /* isolated sources (reference skips them, mssp.cpp:16-18) */
   int32_t reserved;
+  int64_t frontier_edges;    /* sum over steps of the out-degrees of the frontier
+                                vertices a step expands: the edge relaxations of a
+                                push schedule (unweighted batches: per batch, not
+                                per lane; 0 where an engine does not count it) */
 } dawn_stats;
 
 /* ---------------------------------------------------------------- host --- */
@@ -118,6 +122,14 @@ int dawn_csr_from_edges(int32_t n, int64_t count, const int32_t* u,
  * Reference: DAWN::Graph::createGraph (src/graph.cpp:9-77) and
  * DAWN::IO::readGraph* (src/io.cpp:2-192).  Weights must be integral. */
 int dawn_read_mtx(const char* path, int32_t weighted, dawn_csr* out);
+/* The same reader with the edge semantics chosen by the caller, as the
+ * reference's four readers are called directly: as_directed 0 pushes both
+ * directions of every listed pair (DAWN::IO::readGraph / readGraph_Weighted,
+ * src/io.cpp:2-45, 91-143), 1 keeps them as listed (readGraph_Directed /
+ * readGraph_Directed_Weighted, io.cpp:47-89, 145-192), -1 follows the
+ * header's symmetry field (dawn_read_mtx). */
+int dawn_read_mtx_as(const char* path, int32_t weighted, int32_t as_directed,
+                     dawn_csr* out);
 /* Write a CSR as a MatrixMarket coordinate file (general/symmetric). */
 int dawn_write_mtx(const char* path, const dawn_csr* csr);
 void dawn_csr_free(dawn_csr* csr);
@@ -141,6 +153,32 @@ int dawn_graph_create(const dawn_csr* csr, dawn_graph** out);
 int dawn_graph_destroy(dawn_graph* g);
 int dawn_graph_info(const dawn_graph* g, int32_t* n, int64_t* m,
                     int32_t* directed, int32_t* weighted);
+
+/* Graph built on the device: the edge list is turned into the CSR (and the
+ * in-edge CSC of a directed graph) by GPU kernels -- emission of directed
+ * entries, a stable radix sort by (row, column, weight), compaction -- with
+ * the same arrays as dawn_csr_from_edges + dawn_graph_create, bit for bit:
+ * self-loops dropped, symmetrize adds v->u, dedup keeps the minimum weight,
+ * rows sorted by (column, weight).  u, v, w: host or device pointers (w may
+ * be NULL: unweighted); an id outside [0, n) or a negative weight gives
+ * DAWN_ERR_INVALID_ARG.  Reference: DAWN::Matrix::coo2Csr / coo2Csr_Weighted
+ * (src/matrix.cpp:28-82) after DAWN::IO::readGraph* (src/io.cpp:2-192). */
+int dawn_graph_from_edges(int32_t n, int64_t count, const int32_t* u,
+                          const int32_t* v, const int32_t* w,
+                          int32_t symmetrize, int32_t dedup, dawn_graph** out);
+/* The seeded generators above, run on the device (same graphs as
+ * dawn_gen_* + dawn_graph_create, bit for bit). */
+int dawn_graph_gen_er(int32_t n, double avg_degree, uint64_t seed,
+                      int32_t weighted, int32_t wmin, int32_t wmax,
+                      dawn_graph** out);
+int dawn_graph_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                        double c, uint64_t seed, int32_t weighted,
+                        int32_t wmin, int32_t wmax, dawn_graph** out);
+int dawn_graph_gen_grid(int32_t side, double extra_fraction, uint64_t seed,
+                        int32_t weighted, int32_t wmin, int32_t wmax,
+                        dawn_graph** out);
+/* Copies a device graph's CSR to a host dawn_csr (free with dawn_csr_free). */
+int dawn_graph_download(const dawn_graph* g, dawn_csr* out);
 
 /* Single-source unweighted DAWN (BOVM).  dist: n int32, host or device
  * pointer.  Reference: DAWN::BFS_GPU::run (src/algorithm/gpu/bfs.cu:107-129),
@@ -282,6 +320,25 @@ int dawn_sched_trace(const dawn_graph* g, const int32_t* sources,
                      int32_t count, int32_t policy, int32_t* modes,
                      float* features, float* times_ms, int32_t max_rows,
                      int32_t* rows);
+
+/* The same for the single-source BOVM engine (dawn_bfs).  Its learned
+ * policy has its own network (trained on single-source steps), installed
+ * with dawn_sched_set_mlp_single; without one it takes the fixed rule.
+ * Recording: every mode discovers the same levels, so each source is
+ * traversed once per mode with that mode forced, steps launched one at a
+ * time between HIP events (minimum of `repeats`), and once under the fixed
+ * rule (dawn_sched_trace_single: under `policy`) for the features and the
+ * rule's choices; one row per (source, step), isolated sources skipped. */
+int dawn_sched_set_mlp_single(const float* params, const float* feat_mean,
+                              const float* feat_inv_std);
+int dawn_sched_record_single(const dawn_graph* g, const int32_t* sources,
+                             int32_t count, int32_t repeats, float* features,
+                             float* times_ms, int32_t* fixed_modes,
+                             int32_t max_rows, int32_t* rows);
+int dawn_sched_trace_single(const dawn_graph* g, const int32_t* sources,
+                            int32_t count, int32_t policy, int32_t* modes,
+                            float* features, float* times_ms,
+                            int32_t max_rows, int32_t* rows);
 
 /* MLP on the GPU (hand-written HIP kernels).  x: N*8, y: N*3 targets
  * (log step time per mode; the network regresses them), params: 1443.

@@ -77,7 +77,7 @@ def ref():
         l.ref_sssp.restype = c.c_int
         l.ref_sssp.argtypes = [c.c_int, _I32P, _I32P, _F32P, c.c_int, _F32P, c.c_char_p, c.c_int]
         l.ref_closeness.restype = c.c_float
-        l.ref_closeness.argtypes = [c.c_int, _I32P, _I32P, _F32P, c.c_int]
+        l.ref_closeness.argtypes = [c.c_int, _I32P, _I32P, _F32P, c.c_int, c.c_int]
         l.ref_mssp_time.restype = c.c_double
         l.ref_mssp_time.argtypes = [c.c_int, c.c_longlong, _I32P, _I32P, _F32P, _I32P, c.c_int,
                                     c.c_int, _F32P]
@@ -247,13 +247,27 @@ def ref_sssp(row_ptr, col, w, source, threads=1):
     return dist
 
 
-def ref_closeness(row_ptr, col, w, source):
+def ref_closeness(row_ptr, col, w, source, threads=1):
+    """DAWN::CC_CPU::kernel / kernel_Weighted of the reference build (one
+    OpenMP thread by default: the weighted min is unsynchronised)."""
     r = ref()
     rp, cc, ww = _csr(row_ptr, col, w)
     n = len(rp) - 1
     wf = None if ww is None else ww.astype(np.float32)
     return r.ref_closeness(n, _p(rp, ctypes.c_int32), _p(cc, ctypes.c_int32),
-                           None if wf is None else _p(wf, ctypes.c_float), int(source))
+                           None if wf is None else _p(wf, ctypes.c_float), int(source), int(threads))
+
+
+def weighted_closeness_ref(dist_row):
+    """The reference's weighted closeness value from an engine-encoded int32
+    row (src/algorithm/cpu/cc.cpp:132-133): float distances with FLT_MAX
+    for unreached vertices, accumulated in order from 0.0f in float, then
+    (row - 1) / sum in double, returned as float."""
+    d = np.asarray(dist_row)
+    f = np.where(d == INF, np.finfo(np.float32).max, d).astype(np.float32)
+    with np.errstate(over="ignore"):
+        acc = np.add.accumulate(f, dtype=np.float32)[-1] if len(f) else np.float32(0)
+        return np.float32((1.0 * len(d) - 1.0) / np.float64(acc))
 
 
 def ref_mssp_time(row_ptr, col, sources, threads, w=None):

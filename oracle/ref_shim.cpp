@@ -69,7 +69,10 @@ int ref_sssp(int n, int* row_ptr, int* col, float* val, int source,
 }
 
 // DAWN::CC_CPU::kernel / kernel_Weighted (src/algorithm/cpu/cc.cpp:37-142).
-float ref_closeness(int n, int* row_ptr, int* col, float* val, int source) {
+// threads > 0 sets the OpenMP team size first (the weighted kernel's GOVMP
+// min is unsynchronised, so exact values need one thread).
+float ref_closeness(int n, int* row_ptr, int* col, float* val, int source, int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
   float elapsed = 0.0f;
   if (val)
     return DAWN::CC_CPU::kernel_Weighted(row_ptr, col, val, n, source, elapsed);

@@ -135,3 +135,15 @@ def test_isolated_source_exits_zero(tmp_path):
     p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n4 4 1\n2 1\n")
     text = run("bfs_gpu", "-i", str(p), "-s", "3")
     assert "isolated" in text
+
+
+def test_cc_gpu_main_weighted_value(mtx, orc):
+    """CC_GPU::run_Weighted returns the reference's float value
+    (src/algorithm/cpu/cc.cpp:132-133, restated by
+    oracle.weighted_closeness_ref and pinned against the reference build)."""
+    _, _, pw = mtx
+    g = dawn.HostGraph.read_mtx(pw, weighted=True)
+    text = run("cc_gpu", "-i", pw, "-s", "17", "-w", "true")
+    val = float([l for l in text.splitlines() if l.startswith("Closeness")][0].split(":")[1])
+    want = float(orc.weighted_closeness_ref(orc.sssp(g.row_ptr, g.col, g.weight, 17)))
+    assert abs(val - want) <= 5e-6  # the main prints five decimals

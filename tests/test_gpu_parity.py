@@ -226,7 +226,13 @@ def test_lane_words_exact(orc, words):
         assert np.array_equal(sums, osums), p
         assert st["skipped_sources"] == int(skipped.sum())
     _, lv = g.mssp_profile(srcs)
-    assert len(lv) == -(-700 // (64 * words)) and (lv > 0).all()
+    # Isolated sources take no lane; the width is the knob, or less when
+    # fewer lanes are needed.
+    live = int((h.degrees()[srcs] > 0).sum())
+    w = 1
+    while w < words and 64 * w < live:
+        w *= 2
+    assert len(lv) == -(-live // (64 * w)) and (lv > 0).all()
 
 
 @pytest.mark.parametrize("words,variant", [(1, 1), (4, 0), (4, 1), (4, 2), (4, 3), (16, 0), (16, 2), (16, 3)])
@@ -388,14 +394,20 @@ def test_wide_batches_hub_view_exact(orc, directed):
         h = dawn.HostGraph.from_csr(*_directed_csr(h.n, rows[keep], h.col[keep]), directed=True)
     g = dawn.Graph(h)
     g.set_knobs(lane_words=16, hub_order=1)
-    srcs = dawn.gen_sources(h.n, 1100, 14)
+    # 1100 sources with out-edges (two batches: isolated sources take no
+    # lane) and 9 isolated ones interleaved; one duplicate.
+    deg = h.degrees()
+    pool = dawn.gen_sources(h.n, 6000, 14)
+    srcs = pool[deg[pool] > 0][:1100].copy()
     srcs[900] = srcs[5]
+    srcs = np.insert(srcs, np.arange(0, 1100, 123), np.flatnonzero(deg == 0)[:9]).astype(np.int32)
     odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    assert int(skipped.sum()) == 9
     for p in ("fixed", "pull", "push", "bitmap"):
         dist, sums, st = g.mssp(srcs, policy=p, stats=True)
         assert np.array_equal(dist, odist), p
         assert np.array_equal(sums, osums), p
-        assert st["batches"] == 2 and st["skipped_sources"] == int(skipped.sum())
+        assert st["batches"] == 2 and st["skipped_sources"] == 9
 
 
 def _directed_csr(n, u, v):
@@ -443,11 +455,13 @@ def test_closeness_sums_only_wide_batches(orc):
     h = dawn.HostGraph.rmat(15, 8, 23)
     g = dawn.Graph(h)
     g.set_knobs(lane_words=16)
-    srcs = dawn.gen_sources(h.n, 1100, 23)
+    pool = dawn.gen_sources(h.n, 6000, 23)
+    srcs = pool[h.degrees()[pool] > 0][:1100].copy()  # two batches of lanes
+    srcs[7] = int(np.flatnonzero(h.degrees() == 0)[0])  # an isolated source takes none
     odist, osums, _, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
     cc, sums, ref_int, st = g.closeness(srcs, policy="learned", stats=True)
     assert np.array_equal(sums, osums)
-    assert st["batches"] == 2
+    assert st["batches"] == 2 and st["skipped_sources"] == 1
     for i in (0, 1, 555, 1099):
         c, s, r = orc.closeness(odist[i])
         assert cc[i] == c and ref_int[i] == r

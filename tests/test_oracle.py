@@ -314,3 +314,25 @@ def test_certify_path_sums_beyond_int32_are_absent(orc):
         assert np.array_equal(row, good)
     with pytest.raises(ValueError):
         orc.sssp(h.row_ptr, h.col, h.weight, 0)
+
+
+def test_weighted_closeness_restatement_matches_reference_build(orc):
+    """The reference's weighted closeness value (float accumulate of the row,
+    FLT_MAX for unreached vertices, cc.cpp:132-133) restated from an int32
+    row equals DAWN::CC_CPU::kernel_Weighted of the reference build: on a
+    connected weighted grid, and on graphs with one and with several
+    unreachable vertices (about (n-1)/FLT_MAX, then 0)."""
+    if not orc.ref_available():
+        pytest.skip("oracle/_ref not built")
+    cases = [dawn.HostGraph.grid(20, 0.05, 81, weighted=True)]
+    # One unreachable vertex (isolated 39), then three (37..39).
+    for iso in (1, 3):
+        n = 40
+        u = np.arange(n - iso - 1, dtype=np.int32)
+        cases.append(dawn.HostGraph.from_edges(n, u, u + 1, (u % 7 + 1).astype(np.int32)))
+    for h in cases:
+        for s in (0, 5):
+            row = orc.sssp(h.row_ptr, h.col, h.weight, s)
+            want = orc.ref_closeness(h.row_ptr, h.col, h.weight, s)
+            assert orc.weighted_closeness_ref(row) == want, (h.n, s)
+    assert orc.weighted_closeness_ref(orc.sssp(cases[2].row_ptr, cases[2].col, cases[2].weight, 0)) == 0
