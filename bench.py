@@ -163,6 +163,46 @@ def host_cpu():
             "omp_num_threads_env": env or None, "cpu_model": model}
 
 
+SINGLE_MLP_PATH = os.path.join(ROOT, "gal-dawn_amd", "sched_mlp_single.json")
+
+
+def single_source_schedule(dawn, g, h, src, count=16):
+    """Single-source BFS (dawn_bfs) on the same graph from the first `count`
+    sources with out-edges: every mode timed at every step from identical
+    state (dawn_sched_record_single), the fixed rule's and the learned
+    network's choices and the per-step oracle-best; then the end-to-end
+    device time of dawn_bfs over those sources under each policy (best of
+    2).  The learned choices on the recorded steps are the host forward pass
+    of the network over the recorded features, which the decide kernel
+    reproduces on the device.  None without a single-source network."""
+    if not os.path.exists(SINGLE_MLP_PATH):
+        return None
+    w = json.load(open(SINGLE_MLP_PATH))
+    p = np.array(w["params"], np.float32)
+    mu = np.array(w["mean"], np.float32)
+    isd = np.array(w["inv_std"], np.float32)
+    dawn.sched_set_mlp(p, mu, isd, single=True)
+    ss = src[h.degrees()[src] > 0][:count]
+    feat, times, fixed = g.sched_record(ss, repeats=1, single=True, max_rows=8192)
+    learned = dawn.mlp_forward(p, (feat - mu) * isd).argmin(axis=1)
+    idx = np.arange(len(times))
+    out = {"sources": int(len(ss)), "steps": int(len(times)),
+           "oracle_best_ms": float(times.min(axis=1).sum()),
+           "fixed_ms": float(times[idx, fixed].sum()),
+           "learned_ms": float(times[idx, learned].sum())}
+    out["learned_speedup_vs_fixed"] = out["fixed_ms"] / out["learned_ms"]
+    out["learned_regret"] = out["learned_ms"] / out["oracle_best_ms"] - 1.0
+    out["fixed_regret"] = out["fixed_ms"] / out["oracle_best_ms"] - 1.0
+    e2e = {}
+    for pol in ("fixed", "learned"):
+        runs = [sum(g.bfs(int(s), policy=pol, stats=True)[1]["device_ms"] for s in ss)
+                for _ in range(2)]
+        e2e[pol] = min(runs)
+    out["end_to_end_ms"] = e2e
+    out["end_to_end_speedup_vs_fixed"] = e2e["fixed"] / e2e["learned"]
+    return out
+
+
 SUITE = ("er65k", "grid2048", "rmat20", "uneven512")
 
 
@@ -430,11 +470,10 @@ def main():
                             "sources_per_s": args.sources / t, "sums_equal_mssp": True}
     log(f"closeness (sums only): {results['closeness']}")
 
-    # Per-step oracle-best schedule on the first batch of the pass: every
-    # mode timed from identical state (dawn_sched_trace), then each policy's
-    # choices.  Isolated sources take no lane, so the first batch holds the
-    # first 64*W sources with edges.
-    sub = src[h.degrees()[src] > 0][:64 * W]
+    # Per-step oracle-best schedule over every batch of the pass: every
+    # mode timed from identical state at each step (dawn_sched_trace), then
+    # each policy's choices.
+    sub = src
     regret = {}
     for pol in ("fixed", "learned"):
         modes, feat, times = g.sched_trace(sub, policy=pol)
@@ -447,7 +486,8 @@ def main():
                     f"ue=2^{feat[i][3]:.1f} lvl={feat[i][4]:.0f} fixed={dawn.MODE_NAMES[modes[i]]} "
                     f"push={times[i][0]:.2f} pull={times[i][1]:.2f} bitmap={times[i][2]:.2f} ms")
     sched = {
-        "subset_sources": int(len(sub)),
+        "sources": int(len(sub)),
+        "steps": int(len(regret["fixed_modes"])),
         "oracle_best_ms": regret["oracle_best"],
         "fixed_ms": regret["fixed"],
         "learned_ms": regret["learned"],
@@ -457,6 +497,9 @@ def main():
         "learned_network": learned_loaded,
     }
     log(f"schedule: {sched}")
+    single = single_source_schedule(dawn, g, h, src)
+    if single:
+        log(f"single-source schedule: {single}")
 
     # Profiling pass (same kernels, one launch at a time with HIP events on
     # the engine's stream) over every batch: per-kernel average durations.
@@ -568,6 +611,7 @@ def main():
         "learned": results["learned"],
         "closeness": results["closeness"],
         "schedule": sched,
+        "single_source_schedule": single,
         "roofline": roofline,
         "batch_roofline_frac": batch_frac,
         "pass_bytes": pass_bytes,

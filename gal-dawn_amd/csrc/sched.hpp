@@ -75,13 +75,28 @@ __host__ __device__ inline float log2p1(double x) {
 #endif
 }
 
-// Feature vector of the step that expands level `prev` (counters c).
+// Feature vector of the step that expands the level with counters c.
+// Vertices still to reach and their edges: the batched engine counts them
+// per level (vertices missing a live lane, their in-degree sum); the
+// single-source engine's levels are disjoint, so they are n and m minus
+// the levels found so far (the expanded ones, s.work_*, and this one).
 __host__ __device__ inline void sched_features(const LevelCounters& c, int step,
-                                               const SchedParams& p, float* x) {
+                                               const SchedParams& p, const SchedState& s,
+                                               float* x) {
+  double uv, ue;
+  if (p.batched) {
+    uv = (double)c.uv + (double)p.uv_base;
+    ue = (double)c.ue;
+  } else {
+    uv = (double)p.n - (double)s.work_fv - (double)c.fv;
+    ue = (double)p.m - (double)s.work_fe - (double)c.fe;
+    uv = uv > 0 ? uv : 0;
+    ue = ue > 0 ? ue : 0;
+  }
   x[0] = log2p1((double)c.fv);
   x[1] = log2p1((double)c.fe);
-  x[2] = log2p1((double)c.uv + (double)p.uv_base);
-  x[3] = log2p1((double)c.ue);
+  x[2] = log2p1(uv);
+  x[3] = log2p1(ue);
   x[4] = (float)step;
   x[5] = log2p1((double)p.n);
   x[6] = log2p1((double)p.avg_degree);
@@ -175,7 +190,7 @@ __device__ inline void sched_decide(SchedState* sp, const LevelCounters* ring,
         s_done = 1;
       } else {
         float x[8];
-        sched_features(c, s.step + 1, p, x);
+        sched_features(c, s.step + 1, p, s, x);
         for (int i = 0; i < 8; i++) s_x[i] = x[i];
         int mode;
         if (s.force_mode >= 0) {

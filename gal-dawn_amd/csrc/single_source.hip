@@ -866,9 +866,10 @@ int run_ss_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
 // calls of a process are serialised per device by a lock (two calls that
 // overlapped on one GPU would race).  The step loop is launched directly,
 // K decide+expand pairs between reads of the done flag: instantiating a
-// captured graph would allocate.  The scheduler network is used only if it
-// is already on the device (installing it allocates); otherwise the
-// learned policy falls back to the fixed rule, as when no network is set.
+// captured graph would allocate.  The single-source scheduler network is
+// used only if it is already on the device (installing it allocates);
+// otherwise the learned policy falls back to the fixed rule, as when no
+// network is set.
 
 constexpr int32_t kNoAllocMaxN = 1 << 22;
 constexpr int kNoAllocMaxDevices = 64;
@@ -947,7 +948,7 @@ int run_bfs_noalloc(dawn_graph* g, int32_t source, int32_t* dist_out, int32_t po
   p.allow_pull = 1;
   p.avg_degree = g->n ? (float)((double)g->m / g->n) : 0.f;
   p.max_degree = (float)g->max_out_degree;
-  p.mlp = sched_mlp_device_if_present(g->device, &p.mlp_norm);
+  p.mlp = sched_single_mlp_if_present(g->device, &p.mlp_norm);
   SchedTrace tr{(float*)(base + kNaFeat), (int32_t*)(base + kNaMode)};
 
   const unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock, 4096);

@@ -114,3 +114,84 @@ def test_learned_vs_fixed_on_held_out_steps(small_model):
           f"learned {t_learned:.3f} ms, gain {t_fixed / t_learned:.3f}x", flush=True)
     assert t_learned <= 1.05 * t_fixed
     assert t_learned <= 1.25 * best
+
+
+@pytest.fixture(scope="module")
+def single_model():
+    """Single-source steps recorded on small training graphs (every mode
+    timed from identical state) and a network trained on them."""
+    feats, times, fixed = [], [], []
+    for kind, size, seed in [("rmat", 16, 10), ("rmat", 17, 11), ("er", 1 << 17, 12),
+                             ("grid", 384, 13), ("grid", 256, 14)]:
+        if kind == "rmat":
+            h = dawn.HostGraph.rmat(size, 16, seed)
+        elif kind == "er":
+            h = dawn.HostGraph.er(size, 8.0, seed)
+        else:
+            h = dawn.HostGraph.grid(size, 0.01, seed)
+        g = dawn.Graph(h)
+        pool = dawn.gen_sources(h.n, 200, seed)
+        f, t, fx = g.sched_record(pool[h.degrees()[pool] > 0][:6], repeats=2, single=True,
+                                  max_rows=8192)
+        feats.append(f)
+        times.append(t)
+        fixed.append(fx)
+        g.close()
+    X = np.concatenate(feats)
+    T = np.concatenate(times)
+    Y = np.log2(np.maximum(T, 1e-4) * 1e3).astype(np.float32)
+    mean, std = X.mean(0), X.std(0)
+    std[std < 1e-6] = 1
+    Xn = ((X - mean) / std).astype(np.float32)
+    p, loss = dawn.mlp_train(dawn.mlp_init(9), Xn, Y, epochs=300, batch=128, lr=3e-3, seed=9)
+    return p, mean.astype(np.float32), (1 / std).astype(np.float32)
+
+
+def test_single_source_record_consistent(orc):
+    """Every recorded row belongs to a step of a non-isolated source, the
+    fixed rule's recorded choices are those dawn_bfs makes, and the trace
+    under a forced policy reports that mode at every step."""
+    h = dawn.HostGraph.rmat(15, 16, 31)
+    g = dawn.Graph(h)
+    deg = h.degrees()
+    s = int(np.argmax(deg))
+    iso = int(np.flatnonzero(deg == 0)[0])
+    feat, times, fixed = g.sched_record([s, iso], repeats=1, single=True)
+    _, st = g.bfs(s, policy="fixed", stats=True)
+    assert len(times) == st["steps"]
+    assert int((fixed == 1).sum()) == st["mode_steps"]["pull"]
+    assert int((fixed == 0).sum()) == st["mode_steps"]["push"]
+    assert (times > 0).all()
+    modes, _, _ = g.sched_trace([s], policy="push", single=True)
+    assert (modes == 0).all() and len(modes) == len(times)
+
+
+def test_single_source_learned_beats_fixed_on_held_out_steps(single_model, orc):
+    """On held-out graphs the single-source network's choices cost less than
+    the fixed alpha/beta rule's on the same recorded steps, and stay within
+    25% of the per-step oracle-best; dawn_bfs under the installed network
+    gives the oracle's distances."""
+    p, mean, inv_std = single_model
+    tot = {"fixed": 0.0, "learned": 0.0, "best": 0.0}
+    for kind, size, seed in [("rmat", 17, 98), ("grid", 320, 97)]:
+        h = dawn.HostGraph.rmat(size, 16, seed) if kind == "rmat" else dawn.HostGraph.grid(size, 0.01, seed)
+        g = dawn.Graph(h)
+        pool = dawn.gen_sources(h.n, 200, seed)
+        src = pool[h.degrees()[pool] > 0][:6]
+        feat, times, fixed = g.sched_record(src, repeats=2, single=True, max_rows=8192)
+        learned = dawn.mlp_forward(p, (feat - mean) * inv_std).argmin(axis=1)
+        idx = np.arange(len(times))
+        tot["fixed"] += times[idx, fixed].sum()
+        tot["learned"] += times[idx, learned].sum()
+        tot["best"] += times.min(axis=1).sum()
+        dawn.sched_set_mlp(p, mean, inv_std, single=True)
+        try:
+            for s in src[:2]:
+                assert np.array_equal(g.bfs(int(s), policy="learned"), orc.bfs(h.row_ptr, h.col, int(s)))
+        finally:
+            dawn.sched_set_mlp(None, single=True)
+        g.close()
+    print(f"held-out single-source steps: oracle-best {tot['best']:.3f} ms, fixed {tot['fixed']:.3f} ms, "
+          f"learned {tot['learned']:.3f} ms, gain {tot['fixed'] / tot['learned']:.3f}x", flush=True)
+    assert tot["learned"] < tot["fixed"]
+    assert tot["learned"] <= 1.25 * tot["best"]
