@@ -242,7 +242,14 @@ def run_suite(dawn, orc, steps, only=SUITE):
         sssp_ms = best(lambda: g.sssp(int(src[0]), stats=True)[1])
         _, st = g.sssp(int(src[0]), stats=True)
         ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
-        ms_steps = g.mssp(src, want_dist=False, stats=True)[2]["steps"]
+        ms_st = g.mssp(src, want_dist=False, stats=True)[2]
+        ms_steps = ms_st["steps"]
+        # Each mode forced (the rule above picks per step).
+        by_policy = {}
+        for pol in ("push", "pull"):
+            by_policy[pol] = {
+                "sssp_ms": best(lambda: g.sssp(int(src[0]), policy=pol, stats=True)[1]),
+                "mssp16_ms": best(lambda: g.mssp(src, policy=pol, want_dist=False, stats=True)[2])}
         singles_ms = sum(g.sssp(int(x), stats=True)[1]["device_ms"] for x in src)
         ok = bool(np.array_equal(g.sssp(int(src[0])), orc.dijkstra(h.row_ptr, h.col, h.weight, int(src[0]))))
         rows, _ = g.mssp(src[:2])
@@ -261,6 +268,7 @@ def run_suite(dawn, orc, steps, only=SUITE):
                           "sssp_relaxations_per_edge": fe / m if fe else None,
                           "sssp_roofline_frac": b_one / (sssp_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "mssp16_ms": ms_ms, "mssp16_steps": ms_steps,
+                          "mssp16_mode_steps": ms_st["mode_steps"], "forced_modes": by_policy,
                           "mssp16_roofline_frac": b_batch / (ms_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "sssp_x16_ms": singles_ms,
                           "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
@@ -330,6 +338,8 @@ def main():
     ap.add_argument("--seed", type=int, default=99)
     ap.add_argument("--cpu-sample", type=int, default=0, help="sources for the CPU baseline (0: auto)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--host-build", action="store_true",
+                    help="build the graph with the host generator and upload it (default: on the device)")
     ap.add_argument("--retune", action="store_true",
                     help="sweep the launch knobs on the graph instead of loading the tune file")
     ap.add_argument("--tune-out", default=None, help="with --retune: write the tune cache here")
@@ -374,19 +384,32 @@ def main():
         run_suite(dawn, __graft_entry__.load_oracle(), max(1, args.steps), only)
         return
 
-    t0 = time.time()
-    h = dawn.HostGraph.rmat(args.scale, args.edge_factor, args.seed)
-    build_s = time.time() - t0
-    n, m = h.n, h.m
-    log(f"graph RMAT scale {args.scale} ef {args.edge_factor}: n={n} m={m} built in {build_s:.1f}s")
     # Launch knobs measured by the autotuner on this graph class, committed
     # beside the package; graphs created afterwards start from them.
     # (Loaded with --retune too, so that a saved cache keeps the other classes.)
     if os.path.exists(TUNE_PATH):
         dawn.tune_load(TUNE_PATH)
-    t0 = time.time()
-    g = dawn.Graph(h)
-    upload_s = time.time() - t0
+    if args.host_build:
+        t0 = time.time()
+        h = dawn.HostGraph.rmat(args.scale, args.edge_factor, args.seed)
+        build_s = time.time() - t0
+        t0 = time.time()
+        g = dawn.Graph(h)
+        upload_s = time.time() - t0
+        build_where, download_s = "host generator + upload", 0.0
+    else:
+        # Generated and built on the device (edge emission, radix sort,
+        # compaction); the host copy feeds the oracle check and the CPU leg.
+        t0 = time.time()
+        g = dawn.Graph.rmat_device(args.scale, args.edge_factor, args.seed)
+        build_s = time.time() - t0
+        t0 = time.time()
+        h = g.download()
+        download_s = time.time() - t0
+        build_where, upload_s = "device (edge emission, radix sort, compaction)", 0.0
+    n, m = h.n, h.m
+    log(f"graph RMAT scale {args.scale} ef {args.edge_factor}: n={n} m={m} built in {build_s:.2f}s "
+        f"({build_where})")
     src = dawn.gen_sources(n, args.sources, args.seed)
     tune_s = 0.0
     if args.retune:
@@ -417,14 +440,19 @@ def main():
     if args.quick_profile:
         # Learned schedule (deterministic), one batch per call (one stream):
         # the same kernels in the same order every run, so the dispatches of
-        # the counter runs line up with those of the kernel trace, and the
-        # steps are those of the profiling pass below.
+        # the counter runs line up with those of the kernel trace.  The
+        # batches are the timed pass's: the sources of nonzero out-degree in
+        # order of decreasing out-degree (stable), 64*W per batch.
+        deg = h.degrees()
+        live = src[deg[src % n] > 0]
+        live = live[np.argsort(-deg[live % n], kind="stable")]
         dev_ms, steps = 0.0, 0
-        for b0 in range(0, len(src), 64 * W):
-            _, _, st = g.mssp(src[b0:b0 + 64 * W], policy="learned", want_dist=False, stats=True)
+        for b0 in range(0, len(live), 64 * W):
+            _, _, st = g.mssp(live[b0:b0 + 64 * W], policy="learned", want_dist=False, stats=True)
             dev_ms += st["device_ms"]
             steps += st["steps"]
-        print(json.dumps({"quick_profile": True, "device_ms": dev_ms, "steps": steps}))
+        print(json.dumps({"quick_profile": True, "device_ms": dev_ms, "steps": steps,
+                          "batches": -(-len(live) // (64 * W))}))
         return
     copy_gbps = dawn.probe_copy(1 << 30, 5)
     log(f"plain copy: {copy_gbps:.0f} GB/s")
@@ -624,6 +652,8 @@ def main():
         "parity": {"checked_sources": int(len(chk)) if args.check > 0 else 0, "dist_bit_exact": parity,
                    "timed_sums_bit_exact": parity_sums},
         "graph_build_s": build_s,
+        "graph_build": build_where,
+        "graph_download_s": download_s,
         "tune_s": tune_s,
         "knobs": knobs,
         "upload_s": upload_s,

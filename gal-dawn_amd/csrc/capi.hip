@@ -21,7 +21,7 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
 int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
               int64_t* reached, int64_t* edges, int32_t policy, dawn_stats* stats, bool rows);
 int run_wmsbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-               dawn_stats* stats);
+               int32_t policy, dawn_stats* stats);
 int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
                   float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
                   int32_t* rows_out, int32_t policy);
@@ -159,7 +159,7 @@ int dawn_mssp(const dawn_graph* gc, const int32_t* sources, int32_t count, int32
   dawn_graph* g = const_cast<dawn_graph*>(gc);
   if (!g || count < 0 || (count > 0 && !sources)) return DAWN_ERR_INVALID_ARG;
   if (count == 0) return DAWN_OK;
-  if (g->weighted) return run_wmsbfs(g, sources, count, dist, sums, stats);
+  if (g->weighted) return run_wmsbfs(g, sources, count, dist, sums, policy, stats);
   return run_msbfs(g, sources, count, dist, sums, nullptr, nullptr, policy, stats, true);
 }
 
@@ -180,7 +180,7 @@ int dawn_closeness(const dawn_graph* gc, const int32_t* sources, int32_t count,
   std::vector<int64_t> s(count);
   if (count > 0) {
     // Sums only: the unweighted engine skips the distance rows.
-    if (g->weighted) DAWN_TRY(run_wmsbfs(g, sources, count, nullptr, s.data(), stats));
+    if (g->weighted) DAWN_TRY(run_wmsbfs(g, sources, count, nullptr, s.data(), policy, stats));
     else DAWN_TRY(run_msbfs(g, sources, count, nullptr, s.data(), nullptr, nullptr, policy, stats, false));
   }
   for (int32_t i = 0; i < count; i++) {

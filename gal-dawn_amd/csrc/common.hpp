@@ -248,6 +248,59 @@ __device__ __forceinline__ int block_append_n(unsigned int* counter, int k, int*
   return r;
 }
 
+// Queue appends of a workgroup staged in LDS.  A global counter that every
+// wave of the grid bumps serialises at one address: thousands of such
+// atomics per step cost more than the step's own memory traffic.  Waves
+// reserve LDS slots with LDS atomics instead, and the workgroup reserves
+// its global queue range once, in flush().  Appends beyond the LDS
+// capacity go straight to the global queue (wave-aggregated), so nothing
+// is lost on overflow.
+template <int Cap>
+struct LdsQueue {
+  int32_t v[Cap];
+  unsigned n;     // slots reserved (may pass Cap)
+  unsigned base;  // global position of v[0], set in flush()
+
+  // One thread; the caller synchronises the workgroup before any append.
+  __device__ void init() { n = 0u; }
+
+  // Called by all 64 lanes of a wave.
+  __device__ void append(bool pred, int32_t x, unsigned* qlen, int32_t* q_out) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const unsigned cnt = (unsigned)__popcll(m);
+    unsigned b = 0;
+    if (lane_id() == leader) b = atomicAdd(&n, cnt);
+    b = __shfl(b, leader);
+    const unsigned idx = b + (unsigned)__popcll(m & lanes_below_mask());
+    if (b + cnt <= (unsigned)Cap) {
+      if (pred) v[idx] = x;
+      return;
+    }
+    // LDS full: slots [b, Cap) are still filled here (flush copies them),
+    // the rest go to the global queue.
+    const unsigned first_global = b > (unsigned)Cap ? b : (unsigned)Cap;
+    const unsigned ng = b + cnt - first_global;
+    unsigned g = 0;
+    if (lane_id() == leader) g = atomicAdd(qlen, ng);
+    g = __shfl(g, leader);
+    if (pred) {
+      if (idx < (unsigned)Cap) v[idx] = x;
+      else q_out[g + (idx - first_global)] = x;
+    }
+  }
+
+  // Every thread of the workgroup, once, after its last append.
+  __device__ void flush(unsigned* qlen, int32_t* q_out) {
+    __syncthreads();
+    const unsigned cnt = n < (unsigned)Cap ? n : (unsigned)Cap;
+    if (threadIdx.x == 0) base = cnt ? atomicAdd(qlen, cnt) : 0u;
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) q_out[base + i] = v[i];
+  }
+};
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
   for (int o = 32; o >= 1; o >>= 1) {
     unsigned lo = __shfl_xor((unsigned)(x & 0xffffffffu), o);

@@ -123,9 +123,25 @@ struct MsArgs {
   int32_t variant;        // light pull variant (TuneKnobs::pull_variant)
   const int32_t* perm;    // original id -> position (hub-first view), or null
   LevelCounters* ring;
+  unsigned long long* part;  // kRing x kMsStripes counter stripes
   SchedState* st;
   MsBatch* batch;
 };
+
+// Level counters are added per workgroup into one of kMsStripes stripes of
+// the level (stripe = workgroup mod kMsStripes, each on its own 256-byte
+// span): the same-address atomics of thousands of workgroups would
+// serialise at one line (about 12 ns each).  Stripe layout: fv, fe, scout,
+// flanes, uv, ue, alive[16].  The decide kernel sums (ORs) the stripes of
+// the level it reads into the level's ring entry.
+constexpr int kMsStripes = 64;
+constexpr int kMsStripeU64 = 32;
+constexpr int kMsPartU64 = kRing * kMsStripes * kMsStripeU64;
+static_assert(kMsStripes == kWave, "the decide kernel reduces one stripe per lane");
+
+__device__ __forceinline__ unsigned long long* ms_stripe(const MsArgs& a, int lv) {
+  return a.part + ((size_t)(lv % kRing) * kMsStripes + blockIdx.x % kMsStripes) * kMsStripeU64;
+}
 
 struct MsOut {
   unsigned long long* sum;  // per lane, indexed [lane]
@@ -145,6 +161,7 @@ struct MsWorkspace {
   DevBuf<int32_t> iso;  // (row, source) pairs of isolated sources
   int64_t qcap = 0;
   DevBuf<LevelCounters> ring, ring_bak;
+  DevBuf<unsigned long long> part, part_bak;
   DevBuf<SchedState> st;
   DevBuf<MsBatch> batch;
   DevBuf<float> trace_feat;
@@ -158,9 +175,11 @@ struct MsWorkspace {
 };
 
 void ss_invalidate_graphs(SsWorkspace* ws);
+void wms_invalidate_graphs(WmsWorkspace* ws);
 
-// Captured step graphs bake grid sizes and buffer addresses; the autotuner
-// and workspace re-layouts drop them.
+// Captured step graphs bake grid sizes, knobs and buffer addresses (the
+// heavy-chunk lists among them); the autotuner and workspace re-layouts
+// drop them.
 void invalidate_graphs(dawn_graph* g) {
   for (MsWorkspace* ws : {g->ms, g->ms2}) {
     if (!ws) continue;
@@ -168,6 +187,7 @@ void invalidate_graphs(dawn_graph* g) {
     ws->graphs.clear();
   }
   ss_invalidate_graphs(g->ss);
+  wms_invalidate_graphs(g->wms);
 }
 
 void ms_workspace_free(MsWorkspace* ws) {
@@ -259,7 +279,7 @@ template <int W, int NA>
 struct MsAcc {
   unsigned long long fv = 0, fe = 0, scout = 0, lanes = 0, uv = 0, ue = 0;
   uint64_t alive[NA] = {};
-  __device__ void flush(LevelCounters* c, int sub) {
+  __device__ void flush(unsigned long long* c, int sub) {
     constexpr int R = 6 + W;
     __shared__ unsigned long long s_v[kBlock / kWave][R];
     const int w = threadIdx.x / kWave;
@@ -277,18 +297,10 @@ struct MsAcc {
         for (int i = 0; i < 6; i++) t[i] += s_v[j][i];
         for (int i = 6; i < R; i++) t[i] |= s_v[j][i];
       }
-      if (t[0]) {
-        atomicAdd(&c->fv, t[0]);
-        atomicAdd(&c->fe, t[1]);
-        atomicAdd(&c->scout, t[2]);
-        atomicAdd(&c->flanes, t[3]);
-      }
-      if (t[4]) {
-        atomicAdd(&c->uv, t[4]);
-        atomicAdd(&c->ue, t[5]);
-      }
+      for (int i = 0; i < 6; i++)
+        if (t[i]) atomicAdd(&c[i], t[i]);
       for (int i = 0; i < W; i++)
-        if (t[6 + i]) atomicOr((unsigned long long*)&c->alive[i], t[6 + i]);
+        if (t[6 + i]) atomicOr(&c[6 + i], t[6 + i]);
     }
   }
 };
@@ -324,6 +336,8 @@ __global__ void ms_fill_kernel(MsArgs a, MsOut o) {
     a.done[w] = d;
   }
   if (blockIdx.x == 0 && threadIdx.x < kRing) a.ring[threadIdx.x] = LevelCounters{};
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < kMsPartU64; t += gs)
+    a.part[t] = 0ull;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 64 * W; t += gs) {
     o.sum[t] = 0;
     o.reached[t] = 0;
@@ -389,6 +403,15 @@ __global__ void __launch_bounds__(kMaxLanes) ms_seed_kernel(MsArgs a) {
     c.uv = (unsigned long long)(a.n - a.zero_in) - s_c[2];
     c.ue = (unsigned long long)a.m - s_c[3];
     a.ring[0] = c;
+    // Level 0's counters in its stripe 0 (the other stripes are zero).
+    unsigned long long* p0 = a.part;
+    p0[0] = c.fv;
+    p0[1] = c.fe;
+    p0[2] = c.scout;
+    p0[3] = c.flanes;
+    p0[4] = c.uv;
+    p0[5] = c.ue;
+    for (int k = 0; k < 16; k++) p0[6 + k] = c.alive[k];
     SchedState st{};
     st.step = 0;
     st.e2c = a.m;
@@ -399,7 +422,31 @@ __global__ void __launch_bounds__(kMaxLanes) ms_seed_kernel(MsArgs a) {
   }
 }
 
+// One wave: reduces the counter stripes of the last completed level into
+// its ring entry (an assignment, so running it twice is harmless), then
+// decides.
 __global__ void ms_decide_kernel(MsArgs a, SchedParams p, SchedTrace tr) {
+  const SchedState s = *a.st;
+  if (!s.done) {
+    const int slot = s.step % kRing;
+    const unsigned long long* sp = a.part + ((size_t)slot * kMsStripes + lane_id()) * kMsStripeU64;
+    unsigned long long v[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) v[i] = wave_sum_u64(sp[i]);
+    uint64_t al[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) al[k] = wave_or_u64(sp[6 + k]);
+    if (threadIdx.x == 0) {
+      LevelCounters* c = a.ring + slot;
+      c->fv = v[0];
+      c->fe = v[1];
+      c->scout = v[2];
+      c->flanes = v[3];
+      c->uv = v[4];
+      c->ue = v[5];
+      for (int k = 0; k < 16; k++) c->alive[k] = al[k];
+    }
+  }
   sched_decide(a.st, a.ring, p, tr);
 }
 
@@ -609,30 +656,20 @@ __device__ __forceinline__ void pull_light_fused(const MsArgs& a, int lv, const 
       }
     }
   }
-  // Workgroup totals, one set of atomics per workgroup.
+  // Workgroup totals, one set of atomics per workgroup into its stripe.
   const unsigned long long r[6] = {c_fv, wave_sum_u64(c_fe), wave_sum_u64(c_scout),
                                    wave_sum_u64(c_lanes), c_uv, wave_sum_u64(c_ue)};
   if (lane == 0)
     for (int i = 0; i < 6; i++) s_cnt[threadIdx.x / kWave][i] = r[i];
   __syncthreads();
-  LevelCounters* c = a.ring + (lv % kRing);
-  if (threadIdx.x == 0) {
-    unsigned long long t[6] = {};
-    for (int w = 0; w < kBlock / kWave; w++)
-      for (int i = 0; i < 6; i++) t[i] += s_cnt[w][i];
-    if (t[0]) {
-      atomicAdd(&c->fv, t[0]);
-      atomicAdd(&c->fe, t[1]);
-      atomicAdd(&c->scout, t[2]);
-      atomicAdd(&c->flanes, t[3]);
-    }
-    if (t[4]) {
-      atomicAdd(&c->uv, t[4]);
-      atomicAdd(&c->ue, t[5]);
-    }
+  unsigned long long* c = ms_stripe(a, lv);
+  if (threadIdx.x < 6) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / kWave; w++) t += s_cnt[w][threadIdx.x];
+    if (t) atomicAdd(&c[threadIdx.x], t);
   }
   if (threadIdx.x < W && s_alive[threadIdx.x])
-    atomicOr((unsigned long long*)&c->alive[threadIdx.x], (unsigned long long)s_alive[threadIdx.x]);
+    atomicOr(&c[6 + threadIdx.x], (unsigned long long)s_alive[threadIdx.x]);
 }
 
 // Expansion of one step, one kernel per mode (the device scheduler picks
@@ -650,6 +687,9 @@ __device__ __forceinline__ void pull_light_fused(const MsArgs& a, int lv, const 
   const int lane = lane_id();                                               \
   const int h = lane % TPV;                                                 \
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ring[(lv + 1) % kRing] = LevelCounters{}; \
+  if (blockIdx.x == 0)                                                      \
+    for (int i_ = threadIdx.x; i_ < kMsStripes * kMsStripeU64; i_ += blockDim.x) \
+      a.part[(size_t)((lv + 1) % kRing) * kMsStripes * kMsStripeU64 + i_] = 0ull; \
   (void)lane;                                                               \
   (void)h
 
@@ -980,7 +1020,7 @@ __global__ void __launch_bounds__(kBlock) ms_finalize_kernel(MsArgs a) {
   const int lv = st.step;
   uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
   uint32_t* fb = a.fbits + (int64_t)(lv % a.H) * a.fwords;
-  LevelCounters* c_out = a.ring + (lv % kRing);
+  unsigned long long* c_out = ms_stripe(a, lv);
   const int lane = lane_id();
   const int sub = lane % TPV;  // half of the record (W >= 8)
   // Live lanes of the previous level for this thread's words of a vertex
@@ -1112,7 +1152,7 @@ __global__ void __launch_bounds__(kBlock) ms_fold_heavy_kernel(MsArgs a) {
   const int lv = st.step;
   uint64_t* cur = a.hist + (int64_t)(lv % a.H) * a.npad * W;
   uint32_t* fb = a.fbits + (int64_t)(lv % a.H) * a.fwords;
-  LevelCounters* c_out = a.ring + (lv % kRing);
+  unsigned long long* c_out = ms_stripe(a, lv);
   const int lane = lane_id();
   const int sub = lane % TPV;
   uint64_t alive_prev[WPV];
@@ -1564,6 +1604,8 @@ int ensure_ms(dawn_graph* g, MsWorkspace*& slot, int W) {
   DAWN_TRY(ws->qb.alloc((size_t)ws->qcap));
   DAWN_TRY(ws->ring.alloc(kRing));
   DAWN_TRY(ws->ring_bak.alloc(kRing));
+  DAWN_TRY(ws->part.alloc(kMsPartU64));
+  DAWN_TRY(ws->part_bak.alloc(kMsPartU64));
   DAWN_TRY(ws->st.alloc(1));
   DAWN_TRY(ws->batch.alloc(1));
   DAWN_TRY(ws->trace_feat.alloc((size_t)kTraceMax * 8));
@@ -1611,6 +1653,7 @@ MsArgs make_args(dawn_graph* g, MsWorkspace* ws) {
   a.heavy = g->knobs.heavy_degree;
   a.variant = g->knobs.pull_variant;
   a.ring = ws->ring.p;
+  a.part = ws->part.p;
   a.st = ws->st.p;
   a.batch = ws->batch.p;
   return a;
@@ -1988,6 +2031,8 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
                               hipMemcpyDeviceToDevice, s));
       DAWN_HIP(hipMemcpyAsync(ws->ring_bak.p, ws->ring.p, sizeof(LevelCounters) * kRing,
                               hipMemcpyDeviceToDevice, s));
+      DAWN_HIP(hipMemcpyAsync(ws->part_bak.p, ws->part.p, sizeof(unsigned long long) * kMsPartU64,
+                              hipMemcpyDeviceToDevice, s));
       DAWN_HIP(hipMemcpyAsync(ws->done_bak.p, ws->done.p, done_bytes, hipMemcpyDeviceToDevice, s));
       for (int mode = 0; mode < DAWN_NUM_MODES; mode++) {
         float best = 1e30f;
@@ -2000,6 +2045,8 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
                                   hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipMemcpyAsync(ws->ring.p, ws->ring_bak.p, sizeof(LevelCounters) * kRing,
                                   hipMemcpyDeviceToDevice, s));
+          DAWN_HIP(hipMemcpyAsync(ws->part.p, ws->part_bak.p,
+                                  sizeof(unsigned long long) * kMsPartU64, hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipMemcpyAsync(ws->done.p, ws->done_bak.p, done_bytes, hipMemcpyDeviceToDevice, s));
           DAWN_HIP(hipEventRecord(ws->e_a, s));
           launch_step<W>(a, grid, fin_grid, s);

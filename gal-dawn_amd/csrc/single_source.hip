@@ -47,8 +47,20 @@ struct SsArgs {
   int32_t* q;        // 3 * n
   uint32_t* bm;      // 3 * words
   LevelCounters* ring;
+  unsigned long long* part;  // kRing x kStripes counter stripes
   SchedState* st;
 };
+
+// Level counters (fv, fe, scout) are added per workgroup into one of
+// kStripes stripes of the level, each on its own 128-byte line, so the
+// atomics of thousands of workgroups spread over 64 addresses; the decide
+// kernel sums the stripes of the level it reads.
+constexpr int kStripes = 64;
+constexpr int kStripeU64 = 16;  // 128 bytes per stripe
+constexpr int kPartU64 = kRing * kStripes * kStripeU64;
+// Queue entries a workgroup stages in LDS before reserving global space.
+constexpr int kStageCap = 4096;
+static_assert(kStripes == kWave, "the decide kernel sums one stripe per lane");
 
 struct SsWorkspace {
   int32_t n = 0;
@@ -56,6 +68,7 @@ struct SsWorkspace {
   DevBuf<int32_t> dist, q;
   DevBuf<uint32_t> bm;
   DevBuf<LevelCounters> ring;
+  DevBuf<unsigned long long> part;
   DevBuf<SchedState> st;
   DevBuf<float> trace_feat;
   DevBuf<int32_t> trace_mode;
@@ -106,27 +119,63 @@ struct Acc {
     fe += (unsigned)d;
     scout += (unsigned)(d > 0 ? d : 1);
   }
-  __device__ void flush(LevelCounters* c) {
+  // Wave sums into the workgroup's LDS totals (all 64 lanes).
+  __device__ void reduce(unsigned long long* s_tot) {
     unsigned long long a = wave_sum_u64(fv), b = wave_sum_u64(fe), s = wave_sum_u64(scout);
     if (lane_id() == 0 && a) {
-      atomicAdd(&c->fv, a);
-      atomicAdd(&c->fe, b);
-      atomicAdd(&c->scout, s);
-      atomicAdd(&c->flanes, a);
+      atomicAdd(&s_tot[0], a);
+      atomicAdd(&s_tot[1], b);
+      atomicAdd(&s_tot[2], s);
     }
   }
 };
 
-// Records a newly discovered vertex of the level: queue slot (wave
-// aggregated) and counters.  Called by every lane of the wave.
-__device__ __forceinline__ void emit(bool disc, int32_t v, int32_t* q_out,
-                                     unsigned* qlen, Acc& acc,
-                                     const int32_t* row_ptr) {
-  int slot = wave_append(qlen, disc);
-  if (disc) {
-    q_out[slot] = v;
-    acc.add(row_ptr, v);
+// Workgroup-level staging of a step's outputs: the new level's queue
+// entries and its counters.
+struct SsStage {
+  LdsQueue<kStageCap> q;
+  unsigned long long tot[3];
+  __device__ void init() {
+    if (threadIdx.x == 0) {
+      q.init();
+      tot[0] = tot[1] = tot[2] = 0ull;
+    }
+    __syncthreads();
   }
+  // Every thread, once, at the end of the step kernel.
+  __device__ void flush(const SsArgs& a, Acc& acc, int lv, unsigned* qlen, int32_t* q_out) {
+    acc.reduce(tot);
+    q.flush(qlen, q_out);  // synchronises the workgroup first
+    if (threadIdx.x < 3 && tot[threadIdx.x]) {
+      unsigned long long* stripe =
+          a.part + ((size_t)(lv % kRing) * kStripes + blockIdx.x % kStripes) * kStripeU64;
+      atomicAdd(&stripe[threadIdx.x], tot[threadIdx.x]);
+    }
+  }
+};
+
+// Records a newly discovered vertex of the level: staged queue slot and
+// counters.  Called by every lane of the wave.
+__device__ __forceinline__ void emit(bool disc, int32_t v, SsStage& sg, unsigned* qlen,
+                                     int32_t* q_out, Acc& acc, const int32_t* row_ptr) {
+  sg.q.append(disc, v, qlen, q_out);
+  if (disc) acc.add(row_ptr, v);
+}
+
+// Clears the counter stripes of level lv (grid-stride).
+__device__ __forceinline__ void clear_stripes(const SsArgs& a, int lv, int64_t gt, int64_t gs) {
+  unsigned long long* p = a.part + (size_t)(lv % kRing) * kStripes * kStripeU64;
+  for (int64_t i = gt; i < kStripes * kStripeU64; i += gs) p[i] = 0ull;
+}
+
+// Level counters of a seed kernel: stripe 0 of the level (the others are
+// zero after the fill kernel) and the queue length in the ring.
+__device__ __forceinline__ void seed_counters(const SsArgs& a, int lv, const LevelCounters& c) {
+  a.ring[lv % kRing] = c;
+  unsigned long long* p = a.part + (size_t)(lv % kRing) * kStripes * kStripeU64;
+  p[0] = c.fv;
+  p[1] = c.fe;
+  p[2] = c.scout;
 }
 
 // ---------------------------------------------------------------- kernels ---
@@ -137,6 +186,7 @@ __global__ void ss_fill_kernel(SsArgs a, int weighted) {
   for (int64_t i = t0; i < a.n; i += gs) a.dist[i] = kInf;
   for (int64_t i = t0; i < 3 * a.words; i += gs) a.bm[i] = 0u;
   if (t0 < kRing) a.ring[t0] = LevelCounters{};
+  for (int64_t i = t0; i < kPartU64; i += gs) a.part[i] = 0ull;
   (void)weighted;
 }
 
@@ -155,7 +205,7 @@ __global__ void bfs_seed_kernel(SsArgs a, int32_t policy) {
   c.scout = (unsigned)d;
   c.flanes = 1;
   c.qlen = 1;
-  a.ring[0] = c;
+  seed_counters(a, 0, c);
   SchedState st{};
   st.step = 0;
   st.e2c = a.row_ptr[a.n];
@@ -194,7 +244,7 @@ __global__ void sssp_seed_kernel(SsArgs a) {
     c.scout += (unsigned)(d > 0 ? d : 1);
   }
   c.flanes = c.fv;
-  a.ring[1] = c;
+  seed_counters(a, 1, c);
   SchedState st{};
   st.step = 1;
   st.e2c = a.row_ptr[a.n];
@@ -205,7 +255,25 @@ __global__ void sssp_seed_kernel(SsArgs a) {
   *a.st = st;
 }
 
+// One wave: sums the counter stripes of the last completed level into its
+// ring entry (an assignment, so running it twice is harmless), then decides.
 __global__ void ss_decide_kernel(SsArgs a, SchedParams p, SchedTrace tr) {
+  const SchedState s = *a.st;
+  if (!s.done) {
+    const int slot = s.step % kRing;
+    const unsigned long long* stripe =
+        a.part + ((size_t)slot * kStripes + lane_id()) * kStripeU64;
+    const unsigned long long fv = wave_sum_u64(stripe[0]);
+    const unsigned long long fe = wave_sum_u64(stripe[1]);
+    const unsigned long long sc = wave_sum_u64(stripe[2]);
+    if (threadIdx.x == 0) {
+      LevelCounters* c = a.ring + slot;
+      c->fv = fv;
+      c->fe = fe;
+      c->scout = sc;
+      c->flanes = fv;
+    }
+  }
   sched_decide(a.st, a.ring, p, tr);
 }
 
@@ -230,6 +298,9 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = gt; i < a.words; i += gs) bm_nx[i] = 0u;
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
+  clear_stripes(a, lv + 1, gt, gs);
+  __shared__ SsStage sg;
+  sg.init();
   Acc acc;
 
   if (st.mode == DAWN_MODE_PUSH) {
@@ -259,7 +330,7 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
             atomicOr(&bm_out[v >> 5], 1u << (v & 31));
           }
         }
-        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+        emit(disc, v, sg, qlen, q_out, acc, a.row_ptr);
       }
       __syncthreads();
     }
@@ -299,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
         bm_out[base >> 5] = (uint32_t)fm;
         bm_out[(base >> 5) + 1] = (uint32_t)(fm >> 32);
       }
-      emit(found, v, q_out, qlen, acc, a.row_ptr);
+      emit(found, v, sg, qlen, q_out, acc, a.row_ptr);
     }
   } else {  // DAWN_MODE_BITMAP: dense top-down over the frontier bitmap
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
@@ -325,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
             atomicOr(&bm_out[v >> 5], 1u << (v & 31));
           }
         }
-        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+        emit(disc, v, sg, qlen, q_out, acc, a.row_ptr);
       }
       uint64_t hm = __ballot(act && deg >= kHeavyDegree);
       while (hm) {
@@ -343,12 +414,12 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
               atomicOr(&bm_out[v >> 5], 1u << (v & 31));
             }
           }
-          emit(disc, v, q_out, qlen, acc, a.row_ptr);
+          emit(disc, v, sg, qlen, q_out, acc, a.row_ptr);
         }
       }
     }
   }
-  acc.flush(c_out);
+  sg.flush(a, acc, lv, qlen, q_out);
 }
 
 // Relaxation of edge (u -> v, w) with du = current distance of u.  Returns
@@ -391,6 +462,9 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = gt; i < a.words; i += gs) bm_nx[i] = 0u;
   if (gt == 0) a.ring[(lv + 1) % kRing] = LevelCounters{};
+  clear_stripes(a, lv + 1, gt, gs);
+  __shared__ SsStage sg;
+  sg.init();
   Acc acc;
 
   if (st.mode == DAWN_MODE_PUSH) {
@@ -420,7 +494,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
           v = a.col[k];
           disc = relax(a, src, s_du[j], v, a.w[k], bm_out);
         }
-        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+        emit(disc, v, sg, qlen, q_out, acc, a.row_ptr);
       }
       __syncthreads();
     }
@@ -474,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
         bm_out[base >> 5] = (uint32_t)fm;
         bm_out[(base >> 5) + 1] = (uint32_t)(fm >> 32);
       }
-      emit(found, v, q_out, qlen, acc, a.row_ptr);
+      emit(found, v, sg, qlen, q_out, acc, a.row_ptr);
     }
   } else {  // DAWN_MODE_BITMAP (GOVMP shape: dense scan of the frontier)
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
@@ -499,7 +573,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
           v = a.col[beg + j];
           disc = relax(a, src, du, v, a.w[beg + j], bm_out);
         }
-        emit(disc, v, q_out, qlen, acc, a.row_ptr);
+        emit(disc, v, sg, qlen, q_out, acc, a.row_ptr);
       }
       uint64_t hm = __ballot(act && deg >= kHeavyDegree);
       while (hm) {
@@ -515,12 +589,12 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
             v = a.col[k];
             disc = relax(a, src, hdu, v, a.w[k], bm_out);
           }
-          emit(disc, v, q_out, qlen, acc, a.row_ptr);
+          emit(disc, v, sg, qlen, q_out, acc, a.row_ptr);
         }
       }
     }
   }
-  acc.flush(c_out);
+  sg.flush(a, acc, lv, qlen, q_out);
 }
 
 // Sum of out-degree and count of reached vertices (GTEPS numerator).
@@ -552,6 +626,7 @@ int ensure_ss(dawn_graph* g) {
   DAWN_TRY(ws->q.alloc((size_t)3 * g->n));
   DAWN_TRY(ws->bm.alloc((size_t)3 * ws->words));
   DAWN_TRY(ws->ring.alloc(kRing));
+  DAWN_TRY(ws->part.alloc(kPartU64));
   DAWN_TRY(ws->st.alloc(1));
   DAWN_TRY(ws->trace_feat.alloc((size_t)kTraceMax * 8));
   DAWN_TRY(ws->trace_mode.alloc(kTraceMax));
@@ -593,6 +668,7 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
   a.q = ws->q.p;
   a.bm = ws->bm.p;
   a.ring = ws->ring.p;
+  a.part = ws->part.p;
   a.st = ws->st.p;
 
   SchedParams p{};
@@ -806,6 +882,7 @@ int run_ss_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
     a.q = ws->q.p;
     a.bm = ws->bm.p;
     a.ring = ws->ring.p;
+    a.part = ws->part.p;
     a.st = ws->st.p;
     SchedParams p{};
     p.n = g->n;
@@ -880,7 +957,8 @@ constexpr size_t kNaDist = 0;
 constexpr size_t kNaQ = align256(kNaDist + sizeof(int32_t) * (size_t)kNoAllocMaxN);
 constexpr size_t kNaBm = align256(kNaQ + sizeof(int32_t) * 3 * (size_t)kNoAllocMaxN);
 constexpr size_t kNaRing = align256(kNaBm + sizeof(uint32_t) * 3 * (size_t)kNoAllocWords);
-constexpr size_t kNaState = align256(kNaRing + sizeof(LevelCounters) * kRing);
+constexpr size_t kNaPart = align256(kNaRing + sizeof(LevelCounters) * kRing);
+constexpr size_t kNaState = align256(kNaPart + sizeof(unsigned long long) * kPartU64);
 constexpr size_t kNaFeat = align256(kNaState + sizeof(SchedState));
 constexpr size_t kNaMode = align256(kNaFeat + sizeof(float) * 8 * (size_t)kTraceMax);
 constexpr size_t kNaReach = align256(kNaMode + sizeof(int32_t) * (size_t)kTraceMax);
@@ -935,6 +1013,7 @@ int run_bfs_noalloc(dawn_graph* g, int32_t source, int32_t* dist_out, int32_t po
   a.q = (int32_t*)(base + kNaQ);
   a.bm = (uint32_t*)(base + kNaBm);
   a.ring = (LevelCounters*)(base + kNaRing);
+  a.part = (unsigned long long*)(base + kNaPart);
   a.st = (SchedState*)(base + kNaState);
   unsigned long long* reach = (unsigned long long*)(base + kNaReach);
 

@@ -25,7 +25,8 @@
 // wave, split into 64/L groups of L lanes, group g taking edges g, g+64/L,
 // ... and lane b of a group relaxing source lane b.  Improved lanes are
 // OR-ed into act_next[v]; the first improvement appends v to the next
-// queue.  A K-step loop is captured once into a
+// queue (staged per workgroup in LDS, one global reservation per
+// workgroup).  A K-step loop is captured once into a
 // hipGraph; every kernel reads the step and the done flag from device
 // state, so the graph is replayed until the frontier empties.
 #include <hip/hip_runtime.h>
@@ -43,12 +44,19 @@ namespace {
 
 constexpr int kWChunk = 32;  // steps per captured graph
 constexpr int kWLight = 16;  // out-degree below which a queued vertex gets one lane, not a wave
+constexpr int kWStageCap = 4096;  // next-queue entries a workgroup stages in LDS
 
 struct WState {
   int32_t step;      // parity of the current queue = step & 1
   int32_t done;
   unsigned qlen[2];
+  int32_t mode;        // DAWN_MODE_PUSH or DAWN_MODE_PULL for the current step
+  int32_t pull_steps;  // steps taken in pull mode so far
 };
+
+// Pull when the frontier holds at least n / kWPullDiv vertices (policy
+// FIXED / LEARNED); PUSH and PULL policies force one mode.
+constexpr int64_t kWPullDiv = 16;
 
 struct WBatch {
   int32_t nlanes;
@@ -58,9 +66,18 @@ struct WBatch {
 
 struct WArgs {
   int32_t n;
+  int32_t policy;
   const int32_t* row_ptr;
   const int32_t* col;
   const int32_t* w;
+  const int32_t* in_row_ptr;  // in-edges (CSC; the CSR for undirected graphs)
+  const int32_t* in_col;
+  const int32_t* in_w;
+  const int32_t* hc_v;        // in-edge chunks of vertices of in-degree >= heavy
+  const int32_t* hc_beg;
+  const int32_t* hc_end;
+  int64_t num_hc;
+  int32_t heavy;
   int32_t* dist;        // [n][L]
   uint64_t* act;        // [2][n]
   int32_t* q;           // [2][n]
@@ -75,7 +92,11 @@ __global__ void w_fill_kernel(WArgs a) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gs) a.dist[i] = kInf;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * (int64_t)a.n; i += gs)
     a.act[i] = 0ull;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.st = WState{};
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    WState st{};
+    st.mode = a.policy == DAWN_POLICY_PULL ? DAWN_MODE_PULL : DAWN_MODE_PUSH;
+    *a.st = st;
+  }
 }
 
 // Reference initialisation, one thread per source lane (each owns its
@@ -103,7 +124,7 @@ template <int L>
 __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
   constexpr int G = kWave / L;  // edge groups per wave
   const WState st = *a.st;
-  if (st.done) return;
+  if (st.done || st.mode != DAWN_MODE_PUSH) return;
   const int p = st.step & 1;
   const unsigned qn = st.qlen[p];
   const int32_t* q_in = a.q + (int64_t)p * a.n;
@@ -111,6 +132,9 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
   uint64_t* act_in = a.act + (int64_t)p * a.n;
   uint64_t* act_out = a.act + (int64_t)(p ^ 1) * a.n;
   unsigned* qlen_out = &a.st->qlen[p ^ 1];
+  __shared__ LdsQueue<kWStageCap> s_q;
+  if (threadIdx.x == 0) s_q.init();
+  __syncthreads();
   const int lane = lane_id();
   const int b = lane % L;
   const int grp = lane / L;
@@ -180,8 +204,7 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
           }
           if (imp) append = atomicOr((unsigned long long*)&act_out[v], (unsigned long long)imp) == 0ull;
         }
-        const int pos = wave_append(qlen_out, append);
-        if (append) q_out[pos] = v;
+        s_q.append(append, v, qlen_out, q_out);
       }
       if (light) act_in[u] = 0ull;  // consumed
     }
@@ -214,20 +237,176 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
       // step appends it to the next queue.
       bool append = false;
       if (b == 0 && mine) append = atomicOr((unsigned long long*)&act_out[v], mine) == 0ull;
-      const int pos = wave_append(qlen_out, append);
-      if (append) q_out[pos] = v;
+      s_q.append(append, v, qlen_out, q_out);
     }
     if (lane == 0) act_in[u] = 0ull;  // consumed (each u is in the queue once)
   }
+  s_q.flush(qlen_out, q_out);
 }
 
-__global__ void w_advance_kernel(WState* st) {
+// Pull step, for large frontiers: every vertex v takes, per source lane b,
+// the minimum of dist[u][b] + w over its in-edges (u, w) whose tail
+// improved lane b in the last step, and stores the lanes that improve with
+// plain stores (v's record is written by v's thread only; a concurrent
+// reader of it sees the old or the new value of each lane, both path
+// lengths, so the fixed point is unchanged).  Vertices of in-degree >=
+// heavy are done by in-edge chunks, a wave each: per lane minima reduced
+// over the wave, merged with atomicMin (a vertex's chunks may run in
+// different waves).  Light vertices take 16 source lanes at a time from
+// 16-byte record loads.
+template <int L>
+__global__ void __launch_bounds__(kBlock) w_pull_kernel(WArgs a) {
+  constexpr int CH = L < 16 ? L : 16;
+  constexpr int NC = L / CH;
+  constexpr uint64_t kChunk = CH == 64 ? ~0ull : ((1ull << CH) - 1);
+  const WState st = *a.st;
+  if (st.done || st.mode != DAWN_MODE_PULL) return;
+  const int p = st.step & 1;
+  const uint64_t* act_in = a.act + (int64_t)p * a.n;
+  uint64_t* act_out = a.act + (int64_t)(p ^ 1) * a.n;
+  int32_t* q_out = a.q + (int64_t)(p ^ 1) * a.n;
+  unsigned* qlen_out = &a.st->qlen[p ^ 1];
+  __shared__ LdsQueue<kWStageCap> s_q;
+  if (threadIdx.x == 0) s_q.init();
+  __syncthreads();
+  const int lane = lane_id();
+  auto load_chunk = [](const int32_t* q, int32_t (&x)[CH]) {
+#pragma unroll
+    for (int t = 0; t < CH / 4; t++) {
+      const int4 y = reinterpret_cast<const int4*>(q)[t];
+      x[4 * t] = y.x;
+      x[4 * t + 1] = y.y;
+      x[4 * t + 2] = y.z;
+      x[4 * t + 3] = y.w;
+    }
+  };
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
+       base += (int64_t)gridDim.x * kBlock) {  // wave-uniform
+    const int64_t vv = base + lane;
+    const int32_t v = vv < a.n ? (int32_t)vv : 0;
+    int beg = 0, deg = 0;
+    if (vv < a.n) {
+      beg = a.in_row_ptr[v];
+      deg = a.in_row_ptr[v + 1] - beg;
+    }
+    uint64_t imp = 0;
+    if (vv < a.n && deg > 0 && deg < a.heavy) {
+      int32_t* vp = a.dist + (int64_t)v * L;
+#pragma unroll 1
+      for (int c = 0; c < NC; c++) {
+        int32_t best[CH];
+#pragma unroll
+        for (int bl = 0; bl < CH; bl++) best[bl] = kInf;
+        bool any = false;
+        for (int j = 0; j < deg; j++) {
+          const int32_t u = a.in_col[beg + j];
+          const uint64_t m = (act_in[u] >> (c * CH)) & kChunk;
+          if (!m) continue;
+          any = true;
+          const int32_t wt = a.in_w[beg + j];
+          int32_t du[CH];
+          load_chunk(a.dist + (int64_t)u * L + c * CH, du);
+#pragma unroll
+          for (int bl = 0; bl < CH; bl++)
+            if (((m >> bl) & 1ull) && du[bl] != kInf) best[bl] = min(best[bl], minplus(du[bl], wt));
+        }
+        if (!any) continue;
+        int32_t dv[CH];
+        load_chunk(vp + c * CH, dv);
+        uint64_t ci = 0;
+#pragma unroll
+        for (int bl = 0; bl < CH; bl++)
+          if (best[bl] < dv[bl]) {
+            dv[bl] = best[bl];
+            ci |= 1ull << bl;
+          }
+        if (ci) {
+#pragma unroll
+          for (int t = 0; t < CH / 4; t++)
+            reinterpret_cast<int4*>(vp + c * CH)[t] =
+                make_int4(dv[4 * t], dv[4 * t + 1], dv[4 * t + 2], dv[4 * t + 3]);
+          imp |= ci << (c * CH);
+        }
+      }
+      if (imp) act_out[v] = imp;  // v's own slot (heavy vertices are not here)
+    }
+    s_q.append(imp != 0, v, qlen_out, q_out);
+  }
+  // Heavy vertices by in-edge chunks, one wave each.
+  const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t hc = wave; hc < a.num_hc; hc += nwaves) {  // wave-uniform
+    const int32_t v = a.hc_v[hc];
+    const int kb = a.hc_beg[hc], ke = a.hc_end[hc];
+    uint64_t vimp = 0;
+#pragma unroll 1
+    for (int c = 0; c < NC; c++) {
+      int32_t best[CH];
+#pragma unroll
+      for (int bl = 0; bl < CH; bl++) best[bl] = kInf;
+      for (int k = kb + lane; k < ke; k += kWave) {
+        const int32_t u = a.in_col[k];
+        const uint64_t m = (act_in[u] >> (c * CH)) & kChunk;
+        if (!m) continue;
+        const int32_t wt = a.in_w[k];
+        int32_t du[CH];
+        load_chunk(a.dist + (int64_t)u * L + c * CH, du);
+#pragma unroll
+        for (int bl = 0; bl < CH; bl++)
+          if (((m >> bl) & 1ull) && du[bl] != kInf) best[bl] = min(best[bl], minplus(du[bl], wt));
+      }
+      int32_t mine = kInf;
+#pragma unroll
+      for (int bl = 0; bl < CH; bl++) {
+        const int32_t x = wave_min_i32(best[bl]);
+        if (lane == bl) mine = x;
+      }
+      bool improved = false;
+      if (lane < CH && mine != kInf) {
+        int32_t* dp = a.dist + (int64_t)v * L + c * CH + lane;
+        if (mine < *dp) improved = mine < atomicMin(dp, mine);
+      }
+      vimp |= (__ballot(improved) & kChunk) << (c * CH);
+    }
+    bool append = false;
+    if (lane == 0 && vimp) append = atomicOr((unsigned long long*)&act_out[v], (unsigned long long)vimp) == 0ull;
+    s_q.append(append, v, qlen_out, q_out);
+  }
+  s_q.flush(qlen_out, q_out);
+}
+
+// After a pull step: clears the consumed frontier masks (a push step clears
+// each as it consumes it; a pull step reads them from many threads).
+__global__ void w_clear_kernel(WArgs a) {
+  const WState st = *a.st;
+  if (st.done || st.mode != DAWN_MODE_PULL) return;
+  const int p = st.step & 1;
+  const unsigned qn = st.qlen[p];
+  const int32_t* q_in = a.q + (int64_t)p * a.n;
+  uint64_t* act_in = a.act + (int64_t)p * a.n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < qn;
+       i += (int64_t)gridDim.x * blockDim.x)
+    act_in[q_in[i]] = 0ull;
+}
+
+// Ends a step: the next queue becomes current, and the next step's mode is
+// chosen from its size.
+__global__ void w_advance_kernel(WState* st, int32_t n, int32_t policy) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (st->done) return;
   const int p = st->step & 1;
+  if (st->mode == DAWN_MODE_PULL) st->pull_steps += 1;
   st->qlen[p] = 0;
   st->step += 1;
-  if (st->qlen[p ^ 1] == 0) st->done = 1;
+  const unsigned nq = st->qlen[p ^ 1];
+  if (nq == 0) {
+    st->done = 1;
+    return;
+  }
+  int mode = DAWN_MODE_PUSH;
+  if (policy == DAWN_POLICY_PULL) mode = DAWN_MODE_PULL;
+  else if (policy != DAWN_POLICY_PUSH && (int64_t)nq * kWPullDiv >= (int64_t)n) mode = DAWN_MODE_PULL;
+  st->mode = mode;
 }
 
 // Distance rows (lane b at out + b * stride) and per-lane sums: a workgroup
@@ -277,9 +456,15 @@ struct WmsWorkspace {
   DevBuf<unsigned long long> sums;
   WState* h_state = nullptr;
   WBatch* h_batch = nullptr;
-  std::map<int, hipGraphExec_t> graphs;  // by L
+  std::map<int, hipGraphExec_t> graphs;  // by (L, policy)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+void wms_invalidate_graphs(WmsWorkspace* ws) {
+  if (!ws) return;
+  for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+  ws->graphs.clear();
+}
 
 void wms_workspace_free(WmsWorkspace* ws) {
   if (!ws) return;
@@ -318,7 +503,7 @@ int ensure_wms(dawn_graph* g, int L) {
 
 template <int L>
 int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-          dawn_stats* stats) {
+          int32_t policy, dawn_stats* stats) {
   DAWN_TRY(ensure_wms(g, L));
   WmsWorkspace* ws = g->wms;
   hipStream_t s = g->stream;
@@ -326,9 +511,18 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
   if (dist && !dist_dev) DAWN_TRY(ws->rows.alloc((size_t)L * g->n));
   WArgs a{};
   a.n = g->n;
+  a.policy = policy;
   a.row_ptr = g->row_ptr;
   a.col = g->col;
   a.w = g->weight;
+  a.in_row_ptr = g->in_row_ptr;
+  a.in_col = g->in_col;
+  a.in_w = g->in_weight;
+  a.hc_v = g->hc_v;
+  a.hc_beg = g->hc_beg;
+  a.hc_end = g->hc_end;
+  a.num_hc = g->num_hc;
+  a.heavy = g->knobs.heavy_degree;
   a.dist = ws->dist.p;
   a.act = ws->act.p;
   a.q = ws->q.p;
@@ -336,10 +530,12 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
   a.batch = ws->batch.p;
   const unsigned fill_grid = grid_for((int64_t)g->n * L);
   const unsigned step_grid = 2048;  // 8 workgroups per CU, waves stride the queue
+  const unsigned pull_grid = grid_for(std::max<int32_t>(g->n, 1), kBlock, 4096);
   const unsigned fin_grid = grid_for(std::max<int32_t>(g->n, 1), kBlock, 2048);
 
+  const int key = L * 8 + (policy & 7);
   hipGraphExec_t exec = nullptr;
-  auto it = ws->graphs.find(L);
+  auto it = ws->graphs.find(key);
   if (it != ws->graphs.end()) {
     exec = it->second;
   } else {
@@ -347,13 +543,15 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
     DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < kWChunk; i++) {
       w_step_kernel<L><<<step_grid, kBlock, 0, s>>>(a);
-      w_advance_kernel<<<1, 1, 0, s>>>(a.st);
+      w_pull_kernel<L><<<pull_grid, kBlock, 0, s>>>(a);
+      w_clear_kernel<<<step_grid, kBlock, 0, s>>>(a);
+      w_advance_kernel<<<1, 1, 0, s>>>(a.st, a.n, policy);
     }
     DAWN_HIP(hipStreamEndCapture(s, &graph));
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     DAWN_HIP(e);
-    ws->graphs[L] = exec;
+    ws->graphs[key] = exec;
   }
 
   float total_ms = 0.f;
@@ -401,7 +599,8 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
     if (stats) {
       stats->batches += 1;
       stats->steps += ws->h_state->step;
-      stats->mode_steps[DAWN_MODE_PUSH] += ws->h_state->step;
+      stats->mode_steps[DAWN_MODE_PUSH] += ws->h_state->step - ws->h_state->pull_steps;
+      stats->mode_steps[DAWN_MODE_PULL] += ws->h_state->pull_steps;
     }
   }
   if (stats) stats->device_ms += total_ms;
@@ -412,16 +611,16 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
 
 // Weighted multi-source traversal, batches of up to 64 sources.
 int run_wmsbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64_t* sums,
-               dawn_stats* stats) {
+               int32_t policy, dawn_stats* stats) {
   if (!g || !g->weighted || S < 0 || (S > 0 && !sources) || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   if (S == 0) return DAWN_OK;
   std::lock_guard<std::mutex> guard(g->lock);
   DAWN_HIP(hipSetDevice(g->device));
   const int32_t per = std::min<int32_t>(S, 64);
-  if (per <= 8) return run_w<8>(g, sources, S, dist, sums, stats);
-  if (per <= 16) return run_w<16>(g, sources, S, dist, sums, stats);
-  if (per <= 32) return run_w<32>(g, sources, S, dist, sums, stats);
-  return run_w<64>(g, sources, S, dist, sums, stats);
+  if (per <= 8) return run_w<8>(g, sources, S, dist, sums, policy, stats);
+  if (per <= 16) return run_w<16>(g, sources, S, dist, sums, policy, stats);
+  if (per <= 32) return run_w<32>(g, sources, S, dist, sums, policy, stats);
+  return run_w<64>(g, sources, S, dist, sums, policy, stats);
 }
 
 }  // namespace dawn

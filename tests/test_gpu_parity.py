@@ -306,10 +306,15 @@ def test_weighted_batched_mssp_exact(orc, count):
     if count > 5:
         srcs[5] = srcs[2]
     odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, h.weight, srcs)
-    dist, sums, st = g.mssp(srcs, stats=True)
-    assert np.array_equal(dist, odist)
-    assert np.array_equal(sums, osums)
-    assert st["batches"] == -(-count // 64)
+    for p in ("fixed", "push", "pull"):
+        dist, sums, st = g.mssp(srcs, policy=p, stats=True)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+        assert st["batches"] == -(-count // 64)
+        if p == "pull":
+            assert st["mode_steps"]["push"] == 0 and st["mode_steps"]["pull"] > 0
+        if p == "push":
+            assert st["mode_steps"]["pull"] == 0
 
 
 def test_weighted_batched_multigraph_and_isolated(orc):
@@ -330,17 +335,21 @@ def test_weighted_batched_multigraph_and_isolated(orc):
     deg = np.diff(h.row_ptr)
     srcs = np.array([0, 1, 2, 260, 3, 299, int(np.argmax(deg))], np.int32)
     odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, h.weight, srcs)
-    dist, sums, st = g.mssp(srcs, stats=True)
-    assert np.array_equal(dist, odist)
-    assert np.array_equal(sums, osums)
-    assert st["skipped_sources"] == int(skipped.sum()) == 2
+    for p in ("fixed", "push", "pull"):
+        dist, sums, st = g.mssp(srcs, policy=p, stats=True)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+        assert st["skipped_sources"] == int(skipped.sum()) == 2
 
 
 @pytest.mark.parametrize("count", [5, 40, 64])
 def test_weighted_batched_skewed_directed(orc, count):
     """Batched min-plus on a skewed directed graph: light frontier vertices
     (out-degree < 16, one lane each walking their active source lanes) and
-    heavy ones (a wave each) in the same steps; rows equal Dijkstra's."""
+    heavy ones (a wave each) in the same push steps; pull steps over the
+    in-edges (CSC), vertices of in-degree >= heavy_degree by in-edge chunks
+    (a wave each, several chunks per vertex at heavy_degree 16 / 64-edge
+    chunks); rows equal Dijkstra's under push, pull and the rule."""
     log(f"case: weighted directed RMAT scale 12 ef 8 seed 17, {count} sources")
     h = dawn.HostGraph.rmat(12, 8, 17, weighted=True)
     rows = np.repeat(np.arange(h.n, dtype=np.int32), np.diff(h.row_ptr))
@@ -354,10 +363,16 @@ def test_weighted_batched_skewed_directed(orc, count):
     assert (deg >= 16).any() and (deg < 16).any()
     srcs = dawn.gen_sources(hd.n, count, 17)
     srcs[0] = int(np.argmax(deg))
-    dist, sums = g.mssp(srcs)
-    for i in range(count):
-        want = orc.dijkstra(rp, cc, ww, int(srcs[i]))
-        assert np.array_equal(dist[i], want), i
+    want = np.stack([orc.dijkstra(rp, cc, ww, int(s)) for s in srcs])
+    indeg = np.bincount(cc, minlength=hd.n)
+    assert indeg.max() > 2 * 64  # several in-edge chunks per heavy vertex below
+    for knobs in ({}, {"heavy_degree": 16, "pull_chunk": 64}):
+        if knobs:
+            g.set_knobs(**knobs)
+        for p in ("fixed", "push", "pull"):
+            dist, sums = g.mssp(srcs, policy=p)
+            for i in range(count):
+                assert np.array_equal(dist[i], want[i]), (knobs, p, i)
 
 
 @pytest.mark.parametrize("directed", [False, True])
