@@ -67,10 +67,11 @@ def summarize_rocprof(out_dir, profiles_dir):
     trace, FETCH_SIZE and WRITE_SIZE of `--quick-profile`, the same two
     counters of `--probe-only`) into the committed summaries: per-kernel
     times, per-dispatch traffic of the pull and distance kernels, and the
-    copy calibration factors.  Each step of a captured batch graph launches
-    the push, top-down and pull kernels and all but the chosen one return at
-    once, so per-kernel averages are also given over the dispatches that did
-    work (longer than 20 us)."""
+    copy calibration factors.  Each step of the profiling pass launches the
+    chosen mode's kernels and both fold kernels (finalize, fold_heavy), and
+    the fold kernel a mode does not need returns at once, so per-kernel
+    averages are also given over the dispatches that did work (longer than
+    20 us)."""
     import csv
     import glob
 
@@ -101,8 +102,9 @@ def summarize_rocprof(out_dir, profiles_dir):
 
     def traffic(kernel_sub):
         # Dispatches that did work: longer than 20 us in the kernel trace.
-        # The counter runs replay the same batches in the same order (fixed
-        # schedule, one stream), so the same dispatch indices are used there.
+        # The counter runs replay the same batches in the same order (the
+        # profiling pass: learned schedule, one kernel at a time on one
+        # stream), so the same dispatch indices are used there.
         dur = durations_us(kernel_sub)
         fx, wx = per_dispatch(fetch, kernel_sub), per_dispatch(write, kernel_sub)
         if not (len(dur) == len(fx) == len(wx)):
@@ -119,7 +121,7 @@ def summarize_rocprof(out_dir, profiles_dir):
     final = traffic("ms_final_kernel")
     os.makedirs(profiles_dir, exist_ok=True)
     summary = {
-        "workload": "RMAT scale 24 ef 16 seed 99, 4096 sources, learned schedule, one batch per call (bench.py --quick-profile)",
+        "workload": "RMAT scale 24 ef 16 seed 99, 4096 sources, learned schedule, the benchmark's profiling pass (bench.py --quick-profile)",
         "fetch_factor": fetch_factor,
         "write_factor": write_factor,
         "bytes_per_launch": {"expand_pull": pull["fetch_bytes"] + pull["write_bytes"],
@@ -138,6 +140,57 @@ def summarize_rocprof(out_dir, profiles_dir):
                      f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
                      f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} | {act} |")
     return summary, "\n".join(lines)
+
+
+def summarize_rocprof_c2(out_dir):
+    """Kernel table and per-step times of the rocprofv3 kernel trace of
+    `--quick-profile-c2` (one SSSP and one 16-source MSSP on the weighted
+    2048^2 grid, each run twice: the first run captures the step graphs).
+    A captured weighted step launches the push, pull, clear and advance
+    kernels; the mode not chosen returns at once.  Captured chunks end with
+    steps that return at once after the traversal is done."""
+    import csv
+    import glob
+
+    f = glob.glob(os.path.join(out_dir, "prof_c2q", "**", "*kernel_trace.csv"), recursive=True)
+    g = glob.glob(os.path.join(out_dir, "prof_c2q", "**", "*kernel_stats.csv"), recursive=True)
+    trace = sorted(csv.DictReader(open(f[0])), key=lambda r: int(r["Start_Timestamp"]))
+    stats = list(csv.DictReader(open(g[0])))
+
+    def dur(r):
+        return (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+
+    def short(name):
+        return name.replace("dawn::(anonymous namespace)::", "").replace("(anonymous namespace)::", "").split("(")[0]
+
+    lines = ["| kernel | calls | total ms | avg us | max us | working calls (> 20 us) | avg us of working calls |",
+             "|---|---|---|---|---|---|---|"]
+    for r in stats:
+        d = [dur(x) for x in trace if x["Kernel_Name"] == r["Name"]]
+        w = [x for x in d if x > 20.0]
+        lines.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {len(w)} | "
+                     f"{(sum(w) / len(w)) if w else 0:.1f} |")
+    # Per-step times of the second (timed) run of each traversal.
+    ss = [dur(r) for r in trace if "sssp_expand_kernel" in r["Kernel_Name"]]
+    half = len(ss) // 2
+    steps_ss = ss[half:]
+    runs, cur = [], None
+    for r in trace:
+        k = r["Kernel_Name"]
+        if "w_fill_kernel" in k:
+            cur = []
+            runs.append(cur)
+        elif cur is not None and ("w_step_kernel" in k or "w_pull_kernel" in k):
+            cur.append(("push" if "w_step_kernel" in k else "pull", dur(r)))
+        elif "w_final_kernel" in k:
+            cur = None
+    steps_ms = []
+    for i in range(0, len(runs[-1]), 2):
+        (m1, t1), (m2, t2) = runs[-1][i], runs[-1][i + 1]
+        steps_ms.append(("P" if t2 > t1 else "S") + f"{max(t1, t2):.0f}")
+    return "\n".join(lines), {"sssp_step_us": [round(x) for x in steps_ss],
+                              "mssp16_step_us": steps_ms}
 
 
 def host_cpu():
@@ -352,10 +405,20 @@ def main():
                     help="one learned-schedule pass only (for rocprofv3 runs)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the 1 GiB plain copy kernel (rocprofv3 traffic calibration)")
+    ap.add_argument("--summarize-rocprof-c2", default=None, metavar="DIR",
+                    help="print the tables of DIR/prof_c2q (rocprofv3 kernel trace of --quick-profile-c2)")
+    ap.add_argument("--quick-profile-c2", action="store_true",
+                    help="the weighted 2048^2 grid only: one SSSP from the first source and one "
+                         "16-source MSSP, each after one untimed run (for rocprofv3 runs)")
     ap.add_argument("--suite", nargs="?", const="all", default=None,
                     help="the other workloads (er65k, grid2048, rmat20, uneven512: all, or a "
                          "comma-separated subset), one JSON line each, instead of the headline line")
     args = ap.parse_args()
+    if args.summarize_rocprof_c2:
+        table, steps = summarize_rocprof_c2(args.summarize_rocprof_c2)
+        print(table)
+        print(json.dumps(steps))
+        return
     if args.summarize_rocprof:
         summary, table = summarize_rocprof(args.summarize_rocprof, os.path.join(ROOT, "profiles"))
         print(table)
@@ -373,6 +436,19 @@ def main():
         sys.exit(1)
     if args.probe_only:
         print(json.dumps({"copy_gbps": dawn.probe_copy(1 << 30, 3), "bytes": 1 << 30}))
+        return
+    if args.quick_profile_c2:
+        wl = importlib.import_module("gal_dawn_amd.workloads")
+        h, src = wl.grid2048()
+        g = dawn.Graph(h)
+        out = {"quick_profile_c2": True}
+        for name, fn in (("sssp", lambda: g.sssp(int(src[0]), stats=True)[1]),
+                         ("mssp16", lambda: g.mssp(src, want_dist=False, stats=True)[2])):
+            fn()  # captures the step graphs
+            st = fn()
+            out[name] = {"device_ms": st["device_ms"], "steps": st["steps"], "mode_steps": st["mode_steps"]}
+        print(json.dumps(out), flush=True)
+        g.close()
         return
     if args.suite:
         if os.path.exists(TUNE_PATH):
@@ -438,21 +514,15 @@ def main():
     log(f"learned network loaded: {learned_loaded}")
 
     if args.quick_profile:
-        # Learned schedule (deterministic), one batch per call (one stream):
-        # the same kernels in the same order every run, so the dispatches of
-        # the counter runs line up with those of the kernel trace.  The
-        # batches are the timed pass's: the sources of nonzero out-degree in
-        # order of decreasing out-degree (stable), 64*W per batch.
-        deg = h.degrees()
-        live = src[deg[src % n] > 0]
-        live = live[np.argsort(-deg[live % n], kind="stable")]
-        dev_ms, steps = 0.0, 0
-        for b0 in range(0, len(live), 64 * W):
-            _, _, st = g.mssp(live[b0:b0 + 64 * W], policy="learned", want_dist=False, stats=True)
-            dev_ms += st["device_ms"]
-            steps += st["steps"]
-        print(json.dumps({"quick_profile": True, "device_ms": dev_ms, "steps": steps,
-                          "batches": -(-len(live) // (64 * W))}))
+        # The profiling pass of the timed run below (dawn_mssp_profile: the
+        # batches of the pass, every kernel launched on its own on one
+        # stream, learned schedule), so the dispatches of the kernel trace
+        # and of the counter runs line up with each other and with the
+        # per-kernel averages the benchmark line reports.
+        prof, levels = g.mssp_profile(src, policy="learned")
+        print(json.dumps({"quick_profile": True, "kernel_ms": {k: v[0] for k, v in prof.items()},
+                          "launches": {k: v[1] for k, v in prof.items()},
+                          "levels": [int(x) for x in levels]}))
         return
     copy_gbps = dawn.probe_copy(1 << 30, 5)
     log(f"plain copy: {copy_gbps:.0f} GB/s")
@@ -563,9 +633,14 @@ def main():
         "bytes_per_launch": dom_bytes,
     }
     # The pass: every batch's traversal bytes (its rows excluded) plus one
-    # int32 row per source, isolated sources' rows included.
+    # int32 row per source, isolated sources' rows included.  A batch's
+    # bytes are counted at the narrowest record that holds its live lanes
+    # (the last batch of the pass holds the remainder).
     nb = len(levels)
-    pass_bytes = sum(batch_bytes(n, m, float(lv), W) - 256 * W * n for lv in levels) \
+    live_n = int((h.degrees()[src % n] > 0).sum())
+    lanes = [min(64 * W, live_n - 64 * W * b) for b in range(len(levels))]
+    words = [1 << max(0, int(np.ceil(np.log2(max(1, -(-x // 64)))))) for x in lanes]
+    pass_bytes = sum(batch_bytes(n, m, float(lv), wb) - 256 * wb * n for lv, wb in zip(levels, words)) \
         + 4 * n * args.sources
     lt = results["learned"]["device_ms_per_step"] / 1e3
     batch_frac = pass_bytes / lt / 1e9 / HBM_PEAK_GBS

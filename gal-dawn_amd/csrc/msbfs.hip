@@ -1238,8 +1238,14 @@ __global__ void __launch_bounds__(kBlock) ms_fold_heavy_kernel(MsArgs a) {
 // for a PUSH step: one entry per kPushChunk out-edges of each frontier
 // vertex, so the push step's work per entry is bounded whatever the degree.
 __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
-  __shared__ int s_wtot[kBlock / kWave];
-  __shared__ unsigned s_base;
+  // Entries are staged in LDS and the workgroup reserves its queue range
+  // with one atomic per kCompactCap entries (the queue length is one
+  // address: an atomic per tile from every workgroup would serialise).
+  constexpr int kCompactCap = 2048;
+  __shared__ int32_t s_v[kCompactCap];
+  __shared__ int32_t s_b[kCompactCap];
+  __shared__ int s_scan[kBlock / kWave];
+  __shared__ unsigned s_n, s_base;
   const SchedState st = *a.st;
   if (st.done || st.mode != DAWN_MODE_PUSH) return;
   const int lv = st.step;
@@ -1247,21 +1253,51 @@ __global__ void __launch_bounds__(kBlock) ms_compact_kernel(MsArgs a) {
   int32_t* q = a.q;
   int32_t* qb = a.qb;
   unsigned* qlen = &a.ring[(lv - 1) % kRing].qlen;
+  if (threadIdx.x == 0) s_n = 0u;
+  __syncthreads();
+  auto flush = [&]() {  // block-uniform
+    __syncthreads();
+    const unsigned cnt = s_n;
+    if (threadIdx.x == 0) s_base = cnt ? atomicAdd(qlen, cnt) : 0u;
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < cnt; i += kBlock) {
+      q[s_base + i] = s_v[i];
+      qb[s_base + i] = s_b[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+  };
   for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.npad; b0 += (int64_t)gridDim.x * kBlock) {
     const int32_t v = (int32_t)(b0 + threadIdx.x);
     const bool isf = v < a.n && ((fb[v >> 5] >> (v & 31)) & 1u);
     if (!__syncthreads_or(isf)) continue;  // block-uniform
     const int d = isf ? out_deg(a, v) : 0;
     const int k = (d + kPushChunk - 1) / kPushChunk;
-    const int pos = block_append_n(qlen, k, s_wtot, &s_base);
-    if (isf) {
-      const int32_t rb = a.row_ptr[v];
+    int total;
+    const int off = block_exclusive_scan(k, s_scan, &total);
+    if (s_n + (unsigned)total > (unsigned)kCompactCap) flush();  // s_n, total block-uniform
+    const int32_t rb = isf ? a.row_ptr[v] : 0;
+    if (total > kCompactCap) {
+      // More chunks than the stage holds: straight to the queue.
+      if (threadIdx.x == 0) s_base = atomicAdd(qlen, (unsigned)total);
+      __syncthreads();
       for (int i = 0; i < k; i++) {
-        q[pos + i] = v;
-        qb[pos + i] = rb + i * kPushChunk;
+        q[s_base + off + i] = v;
+        qb[s_base + off + i] = rb + i * kPushChunk;
       }
+    } else {
+      const unsigned at = s_n;
+      for (int i = 0; i < k; i++) {
+        s_v[at + off + i] = v;
+        s_b[at + off + i] = rb + i * kPushChunk;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) s_n = at + (unsigned)total;
     }
+    __syncthreads();
   }
+  flush();
 }
 
 // Rows of isolated sources: INF except 0 at the source.  rs holds (row,
@@ -1737,9 +1773,13 @@ void launch_pull(const MsArgs& a, unsigned grid, hipStream_t s) {
   }
 }
 
+// Bitmap -> queue compaction: a streaming scan, so few workgroups (each
+// reserves its queue range with one atomic on the queue length).
+inline unsigned compact_grid(unsigned grid) { return std::min(grid, 1024u); }
+
 template <int W>
 void launch_step(const MsArgs& a, unsigned grid, unsigned fin_grid, hipStream_t s) {
-  ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
+  ms_compact_kernel<<<compact_grid(grid), kBlock, 0, s>>>(a);
   ms_push_kernel<W><<<grid, kBlock, 0, s>>>(a);
   ms_topdown_kernel<W><<<grid, kBlock, 0, s>>>(a);
   launch_pull<W>(a, grid, s);
@@ -2130,7 +2170,7 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
       const int slot = st.mode == DAWN_MODE_PUSH ? 3 : st.mode == DAWN_MODE_PULL ? 4 : 5;
       DAWN_TRY(timed(slot, [&] {
         if (st.mode == DAWN_MODE_PUSH) {
-          ms_compact_kernel<<<grid, kBlock, 0, s>>>(a);
+          ms_compact_kernel<<<compact_grid(grid), kBlock, 0, s>>>(a);
           ms_push_kernel<W><<<grid, kBlock, 0, s>>>(a);
         } else if (st.mode == DAWN_MODE_PULL) {
           launch_pull<W>(a, grid, s);

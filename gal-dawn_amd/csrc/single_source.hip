@@ -60,6 +60,9 @@ constexpr int kStripeU64 = 16;  // 128 bytes per stripe
 constexpr int kPartU64 = kRing * kStripes * kStripeU64;
 // Queue entries a workgroup stages in LDS before reserving global space.
 constexpr int kStageCap = 4096;
+// Queue length of a level whose queue was not written (a pull step writes
+// the bitmap only): a top-down step that follows reads the bitmap instead.
+constexpr unsigned kNoQueue = 0xFFFFFFFFu;
 static_assert(kStripes == kWave, "the decide kernel sums one stripe per lane");
 
 struct SsWorkspace {
@@ -275,6 +278,17 @@ __global__ void ss_decide_kernel(SsArgs a, SchedParams p, SchedTrace tr) {
     }
   }
   sched_decide(a.st, a.ring, p, tr);
+  if (threadIdx.x == 0) {
+    SchedState* sp = a.st;
+    if (!sp->done && sp->mode == DAWN_MODE_PUSH &&
+        a.ring[(sp->step + kRing - 1) % kRing].qlen == kNoQueue) {
+      // The previous level has no queue: the same top-down step from its
+      // bitmap.
+      sp->mode = DAWN_MODE_BITMAP;
+      sp->last_mode = DAWN_MODE_BITMAP;
+      if (sp->step < kTraceMax && tr.mode) tr.mode[sp->step] = DAWN_MODE_BITMAP;
+    }
+  }
 }
 
 // One BOVM step: discovers level lv = st->step.
@@ -335,6 +349,7 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
       __syncthreads();
     }
   } else if (st.mode == DAWN_MODE_PULL) {
+    if (gt == 0) *qlen = kNoQueue;  // the level's bitmap only
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
@@ -370,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(SsArgs a) {
         bm_out[base >> 5] = (uint32_t)fm;
         bm_out[(base >> 5) + 1] = (uint32_t)(fm >> 32);
       }
-      emit(found, v, sg, qlen, q_out, acc, a.row_ptr);
+      if (found) acc.add(a.row_ptr, v);  // bitmap only: no queue entries
     }
   } else {  // DAWN_MODE_BITMAP: dense top-down over the frontier bitmap
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
@@ -500,7 +515,9 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
     }
   } else if (st.mode == DAWN_MODE_PULL) {
     // Every vertex takes the minimum over its in-edges from frontier
-    // vertices; each vertex is written only by its own lane.
+    // vertices; each vertex is written only by its own lane.  Bitmap only:
+    // no queue entries.
+    if (gt == 0) *qlen = kNoQueue;
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
     for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); base < a.n;
          base += wstride) {
@@ -548,7 +565,7 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
         bm_out[base >> 5] = (uint32_t)fm;
         bm_out[(base >> 5) + 1] = (uint32_t)(fm >> 32);
       }
-      emit(found, v, sg, qlen, q_out, acc, a.row_ptr);
+      if (found) acc.add(a.row_ptr, v);  // bitmap only: no queue entries
     }
   } else {  // DAWN_MODE_BITMAP (GOVMP shape: dense scan of the frontier)
     const int64_t wstride = (int64_t)gridDim.x * kBlock;
@@ -597,9 +614,15 @@ __global__ void __launch_bounds__(kBlock) sssp_expand_kernel(SsArgs a) {
   sg.flush(a, acc, lv, qlen, q_out);
 }
 
-// Sum of out-degree and count of reached vertices (GTEPS numerator).
+// Sum of out-degree and count of reached vertices (GTEPS numerator):
+// wave sums, then one pair of atomics per workgroup (launched with at most
+// kReachBlocks workgroups: the two counters are single addresses).
+constexpr unsigned kReachBlocks = 512;
 __global__ void reach_kernel(const int32_t* dist, const int32_t* row_ptr, int32_t n,
                              unsigned long long* out) {
+  __shared__ unsigned long long s_t[2];
+  if (threadIdx.x == 0) s_t[0] = s_t[1] = 0ull;
+  __syncthreads();
   unsigned long long cnt = 0, edges = 0;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
@@ -611,9 +634,11 @@ __global__ void reach_kernel(const int32_t* dist, const int32_t* row_ptr, int32_
   cnt = wave_sum_u64(cnt);
   edges = wave_sum_u64(edges);
   if (lane_id() == 0 && cnt) {
-    atomicAdd(&out[0], cnt);
-    atomicAdd(&out[1], edges);
+    atomicAdd(&s_t[0], cnt);
+    atomicAdd(&s_t[1], edges);
   }
+  __syncthreads();
+  if (threadIdx.x < 2 && s_t[threadIdx.x]) atomicAdd(&out[threadIdx.x], s_t[threadIdx.x]);
 }
 
 int ensure_ss(dawn_graph* g) {
@@ -752,7 +777,7 @@ int run_single_source(dawn_graph* g, int kind, int32_t source, int32_t* dist_out
   const int first_step = kind == 0 ? 1 : 2;
   if (stats) {
     DAWN_HIP(hipMemsetAsync(ws->reach.p, 0, 2 * sizeof(unsigned long long), s));
-    reach_kernel<<<grid_for(g->n), kBlock, 0, s>>>(ws->dist.p, g->row_ptr, g->n, ws->reach.p);
+    reach_kernel<<<grid_for(g->n, kBlock, kReachBlocks), kBlock, 0, s>>>(ws->dist.p, g->row_ptr, g->n, ws->reach.p);
     unsigned long long rr[2];
     DAWN_HIP(hipMemcpyAsync(rr, ws->reach.p, sizeof rr, hipMemcpyDeviceToHost, s));
     // Trace entries 1..levels hold the modes of the expanding steps.
@@ -1057,7 +1082,7 @@ int run_bfs_noalloc(dawn_graph* g, int32_t source, int32_t* dist_out, int32_t po
   if (stats) {
     const int levels = h_state.levels;
     zero_u64_kernel<<<1, 64, 0, s>>>(reach, 2);
-    reach_kernel<<<grid_for(g->n), kBlock, 0, s>>>(a.dist, g->row_ptr, g->n, reach);
+    reach_kernel<<<grid_for(g->n, kBlock, kReachBlocks), kBlock, 0, s>>>(a.dist, g->row_ptr, g->n, reach);
     DAWN_HIP(hipGetLastError());
     unsigned long long rr[2];
     DAWN_HIP(hipMemcpyAsync(rr, reach, sizeof rr, hipMemcpyDeviceToHost, s));

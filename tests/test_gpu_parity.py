@@ -62,7 +62,11 @@ def test_bfs_fixed_schedule_matches_gapbs(er_c1, orc):
         d, st = g.bfs(s, policy="fixed", stats=True)
         assert st["steps"] == len(modes)
         assert st["mode_steps"]["pull"] == int((modes == 1).sum())
-        assert st["mode_steps"]["push"] == int((modes == 0).sum())
+        # A pull step writes its level's bitmap only, so the top-down step
+        # right after one reads the bitmap (BITMAP); the others the queue.
+        after_pull = int(((modes[1:] == 0) & (modes[:-1] == 1)).sum())
+        assert st["mode_steps"]["bitmap"] == after_pull
+        assert st["mode_steps"]["push"] == int((modes == 0).sum()) - after_pull
 
 
 def test_mssp_c1_batched_equals_oracle(er_c1, orc):

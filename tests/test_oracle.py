@@ -213,7 +213,42 @@ def test_oracle_matches_reference_build_sssp_and_closeness(orc):
         assert ref_int == orc.ref_closeness(hu.row_ptr, hu.col, None, s)
 
 
-# ---- MLP oracle: analytic gradient vs finite differences ---------------------
+@ref_only
+def test_oracle_multigraph_seeding_matches_reference_build():
+    """A multigraph whose sources list duplicate out-edges with different
+    weights: the reference seeds dist[v] with each listed weight in row
+    order (the last listed wins; rows are sorted by (column, weight), so the
+    larger), never relaxes the source, and relaxes from there
+    (src/algorithm/cpu/sssp.cpp:40-46).  The oracle gives the reference
+    build's distances (its unreached sentinel, about 3.4e38, <-> INT32_MAX)."""
+    import oracle as orc_mod
+    rng = np.random.default_rng(21)
+    n = 240
+    u = rng.integers(0, 200, 700).astype(np.int32)  # 200..239 unreachable
+    v = rng.integers(0, 200, 700).astype(np.int32)
+    w = rng.integers(1, 60, 700).astype(np.int32)
+    src = np.array([0, 3, 7], np.int32)
+    # Every edge at the sources repeated three times with other weights.
+    at = np.isin(u, src)
+    k = int(at.sum())
+    u = np.concatenate([u] + [u[at]] * 3)
+    v = np.concatenate([v] + [v[at]] * 3)
+    w = np.concatenate([w] + [rng.integers(1, 60, k).astype(np.int32) for _ in range(3)])
+    h = dawn.HostGraph.from_edges(n, u, v, w, symmetrize=True, dedup=False)
+    dup = differs = 0
+    for s in src:
+        seg = slice(h.row_ptr[s], h.row_ptr[s + 1])
+        cols = h.col[seg]
+        dup += len(cols) - len(np.unique(cols))
+        r = orc_mod.ref_sssp(h.row_ptr, h.col, h.weight, int(s))
+        o = orc_mod.sssp(h.row_ptr, h.col, h.weight, int(s))
+        reached = o != INF
+        assert np.array_equal(reached, r < 3e38)  # the reference's large sentinel
+        assert np.array_equal(o[reached].astype(np.float32), r[reached])
+        differs += int((o != orc_mod.dijkstra(h.row_ptr, h.col, h.weight, int(s))).sum())
+    assert dup > 10  # the case is exercised
+    # ... and tells the reference's seeding from true shortest paths.
+    assert differs > 0
 
 def test_mlp_oracle_gradient_finite_differences(orc):
     rng = np.random.default_rng(0)
