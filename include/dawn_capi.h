@@ -201,8 +201,9 @@ int dawn_sssp(const dawn_graph* g, int32_t source, int32_t* dist,
  * vertices: larger ones return DAWN_ERR_SCRATCH_LIMIT.  The call allocates
  * nothing (no hipMalloc / hipFree, no captured graph), on the graph's
  * device and stream, and returns when the traversal is done.  The learned
- * policy uses the scheduler network only if it is already on the device
- * (another entry point has used it there); otherwise the fixed rule.
+ * policy uses the single-source scheduler network (dawn_sched_set_mlp_single)
+ * only if it is already on the device (dawn_bfs has used it there);
+ * otherwise the fixed rule.
  * stats->device_ms is host wall time of the traversal.  All calls on one
  * GPU share its region: the calls of a process are serialised per device
  * (calls that overlapped on one GPU would race). */
