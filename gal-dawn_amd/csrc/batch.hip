@@ -126,7 +126,8 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
                                int* s_scan, unsigned* s_cnt) {
   const int32_t n = G.n;
   const int64_t mn = a.max_n;
-  int32_t* drow = a.dist + G.dist_off;  // lane b row: drow + b * n
+  // Lane b of the task is source lane0 + b of the graph: row drow + b * n.
+  int32_t* drow = a.dist + G.dist_off + (int64_t)T.lane0 * n;
   const uint64_t lanemask = T.lanes >= 64 ? ~0ull : ((1ull << T.lanes) - 1);
   for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
     seen[v] = ~lanemask;
@@ -380,10 +381,17 @@ extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
       hs[(size_t)i * spg + j] = (int32_t)s;
       if (stats && g->h_row_ptr[s] == g->h_row_ptr[s + 1]) stats->skipped_sources += 1;
     }
-    if (g->weighted)
+    if (g->weighted) {
       for (int32_t j = 0; j < spg; j++) ht.push_back(BatchTask{i, j, 1, 0});
-    else
-      ht.push_back(BatchTask{i, 0, spg, 0});
+    } else {
+      // The sources of a graph share one task as lanes of a word, except
+      // on the largest graphs, whose tasks would be the batch's critical
+      // path: from 500k edges two tasks of half the lanes, from 2M edges a
+      // task per source (512 graphs of 1k-130k vertices, 8 sources each:
+      // 31.5 -> 25.5 ms).
+      const int32_t per = g->m >= 2000000 ? 1 : g->m >= 500000 ? std::max(1, spg / 2) : spg;
+      for (int32_t j = 0; j < spg; j += per) ht.push_back(BatchTask{i, j, std::min(per, spg - j), 0});
+    }
   }
   // Largest graphs first: the long tasks start early.
   std::stable_sort(ht.begin(), ht.end(), [&](const BatchTask& x, const BatchTask& y) {

@@ -49,3 +49,22 @@ def test_batch_c4_full_weighted(orc):
     print("case: 512 uneven graphs (seed 4), 8 sources each, int32 weights", flush=True)
     graphs, src, _ = wl.uneven(512, seed=4, weighted=True)
     check_batch(orc, graphs, src, 8, True)
+
+
+def test_batch_split_tasks_duplicates_isolated(orc):
+    """Graphs of >= 500k and >= 2M edges run as two tasks of 4 lanes and as
+    one task per source; duplicate and isolated sources inside split tasks."""
+    print("case: RMAT-17 (one task per source), RMAT-16 (two tasks), ER 2000", flush=True)
+    big = dawn.HostGraph.rmat(17, 16, 41)
+    mid = dawn.HostGraph.rmat(16, 16, 42)
+    small = dawn.HostGraph.er(2000, 4.0, 43)
+    assert big.m >= 2000000 and 500000 <= mid.m < 2000000
+    iso = int(np.flatnonzero(big.degrees() == 0)[0])
+    s_big = dawn.gen_sources(big.n, 8, 41)
+    s_big[3] = s_big[1]
+    s_big[6] = iso
+    s_mid = dawn.gen_sources(mid.n, 8, 42)
+    s_mid[5] = s_mid[4]
+    s_small = dawn.gen_sources(small.n, 8, 43)
+    st = check_batch(orc, [small, big, mid], np.concatenate([s_small, s_big, s_mid]), 8, False)
+    assert st["batches"] == 1 + 8 + 2  # tasks
