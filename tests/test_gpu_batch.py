@@ -68,3 +68,26 @@ def test_batch_split_tasks_duplicates_isolated(orc):
     s_small = dawn.gen_sources(small.n, 8, 43)
     st = check_batch(orc, [small, big, mid], np.concatenate([s_small, s_big, s_mid]), 8, False)
     assert st["batches"] == 1 + 8 + 2  # tasks
+
+
+def test_batch_weighted_multigraph_and_directed(orc):
+    """Weighted tasks (near-far relaxation) on a multigraph whose sources
+    list duplicate edges with other weights (the last listed seeds) and on a
+    directed graph with weights spanning three orders of magnitude."""
+    rng = np.random.default_rng(17)
+    n = 3000
+    u = rng.integers(0, n, 20000).astype(np.int32)
+    v = rng.integers(0, n, 20000).astype(np.int32)
+    w = rng.integers(1, 1000, 20000).astype(np.int32)
+    at = u < 40
+    u = np.concatenate([u, u[at], u[at]])
+    v = np.concatenate([v, v[at], v[at]])
+    w = np.concatenate([w, rng.integers(1, 1000, int(at.sum())).astype(np.int32),
+                        rng.integers(1, 1000, int(at.sum())).astype(np.int32)])
+    multi = dawn.HostGraph.from_edges(n, u, v, w, symmetrize=True, dedup=False)
+    du = rng.integers(0, 5000, 40000).astype(np.int32)
+    dv = rng.integers(0, 5000, 40000).astype(np.int32)
+    dw = (10 ** rng.integers(0, 3, 40000) * rng.integers(1, 10, 40000)).astype(np.int32)
+    directed = dawn.HostGraph.from_edges(5000, du, dv, dw, symmetrize=False, dedup=True)
+    src = np.concatenate([np.arange(0, 40, 5, dtype=np.int32), dawn.gen_sources(5000, 8, 17)])
+    check_batch(orc, [multi, directed], src, 8, True)
