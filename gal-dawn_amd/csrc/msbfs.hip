@@ -270,8 +270,8 @@ __device__ __forceinline__ uint32_t even_bits(uint64_t x) {
 }
 
 // Per-thread counters of a level, reduced over the workgroup and flushed
-// with one set of atomics per workgroup (same-address atomics from every
-// wave serialise in L2).  alive holds NA of the W lane words: words 0..W-1
+// with one set of atomics per workgroup into the workgroup's counter stripe
+// of the level.  alive holds NA of the W lane words: words 0..W-1
 // (NA = W), or words sub*NA .. sub*NA + NA-1 when W/NA threads share a
 // record and this thread holds part `sub`.  flush() must be called by every
 // thread.

@@ -146,8 +146,8 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
   // source lanes that hold an active lane of u: 16-byte loads of u's and
   // the neighbour's distances (u's held in registers when L <= 16), atomics
   // only for the lanes that improve, and one act_next OR per edge.  The
-  // wave loops until its busiest lane is done, so the appends stay
-  // wave-aggregated.
+  // wave loops until its busiest lane is done, so every lane of the wave
+  // takes part in each (LDS-staged) append.
   {
     constexpr int CH = L < 16 ? L : 16;
     constexpr int NC = L / CH;
