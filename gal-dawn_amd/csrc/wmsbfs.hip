@@ -42,7 +42,7 @@ namespace dawn {
 
 namespace {
 
-constexpr int kWChunk = 32;  // steps per captured graph
+constexpr int kWChunk = 32;  // steps per captured graph before a traversal's depth is known
 constexpr int kWLight = 16;  // out-degree below which a queued vertex gets one lane, not a wave
 constexpr int kWStageCap = 4096;  // next-queue entries a workgroup stages in LDS
 
@@ -456,7 +456,8 @@ struct WmsWorkspace {
   DevBuf<unsigned long long> sums;
   WState* h_state = nullptr;
   WBatch* h_batch = nullptr;
-  std::map<int, hipGraphExec_t> graphs;  // by (L, policy)
+  std::map<int, hipGraphExec_t> graphs;  // by (L, policy, steps per graph)
+  int32_t last_steps = 0;                // steps the last batch took
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -534,7 +535,11 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
   const unsigned pull_grid = grid_for(std::max<int32_t>(g->n, 1), kBlock, 4096);
   const unsigned fin_grid = grid_for(std::max<int32_t>(g->n, 1), kBlock, 2048);
 
-  const int key = L * 8 + (policy & 7);
+  // Steps per captured graph: what the last batch took (+ the one that
+  // finds the frontier empty), rounded up to a multiple of 8, so that one
+  // graph launch and one read of the done flag usually suffice.
+  const int chunk = ws->last_steps > 0 ? std::min(256, (ws->last_steps + 1 + 7) & ~7) : kWChunk;
+  const int key = (chunk * 128 + L) * 8 + (policy & 7);
   hipGraphExec_t exec = nullptr;
   auto it = ws->graphs.find(key);
   if (it != ws->graphs.end()) {
@@ -542,7 +547,7 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
   } else {
     hipGraph_t graph;
     DAWN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < kWChunk; i++) {
+    for (int i = 0; i < chunk; i++) {
       w_step_kernel<L><<<step_grid, kBlock, 0, s>>>(a);
       w_pull_kernel<L><<<pull_grid, kBlock, 0, s>>>(a);
       w_clear_kernel<<<step_grid, kBlock, 0, s>>>(a);
@@ -575,11 +580,11 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
     int64_t launched = 0;
     for (;;) {
       DAWN_HIP(hipGraphLaunch(exec, s));
-      launched += kWChunk;
+      launched += chunk;
       DAWN_HIP(hipMemcpyAsync(ws->h_state, ws->st.p, sizeof(WState), hipMemcpyDeviceToHost, s));
       DAWN_HIP(hipStreamSynchronize(s));
       if (ws->h_state->done) break;
-      if (launched > (int64_t)g->n + 2 * kWChunk) return DAWN_ERR_HIP;
+      if (launched > (int64_t)g->n + 2 * chunk) return DAWN_ERR_HIP;
     }
     int32_t* out = dist ? (dist_dev ? dist + (int64_t)b0 * g->n : ws->rows.p) : nullptr;
     w_zero_kernel<<<1, 64, 0, s>>>(ws->sums.p, 64);
@@ -597,6 +602,7 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
                          hipMemcpyDeviceToHost));
     for (int i = 0; i < lanes; i++)
       if (sums) sums[b0 + i] = (int64_t)h_sums[i];
+    ws->last_steps = ws->h_state->step;
     if (stats) {
       stats->batches += 1;
       stats->steps += ws->h_state->step;
