@@ -276,7 +276,9 @@ def run_suite(dawn, orc, steps, only=SUITE):
         h, src = wl.er65k()
         g = dawn.Graph(h)
         bfs_ms = best(lambda: g.bfs(int(src[0]), stats=True)[1])
+        bfs_edges = g.bfs(int(src[0]), stats=True)[1]["edges_reached"]
         ms_ms = best(lambda: g.mssp(src, want_dist=False, stats=True)[2])
+        ms_edges = g.mssp(src, want_dist=False, stats=True)[2]["edges_reached"]
         dist, _ = g.mssp(src)
         ok = bool(np.array_equal(dist, orc.mssp(h.row_ptr, h.col, None, src)[0]))
         t_cpu, _ = orc.ref_mssp_time(h.row_ptr, h.col, src, threads)
@@ -284,7 +286,9 @@ def run_suite(dawn, orc, steps, only=SUITE):
         # of BFS_CPU::BFS, src/algorithm/cpu/bfs.cpp:23-77), best of 3.
         t_dense = min(orc.time_bfs_dense(h.row_ptr, h.col, int(src[0]), threads) for _ in range(3))
         print(json.dumps({"config": "er65k: ER n=65,536 deg 8 seed 1, source 0 + 64", "bfs_ms": bfs_ms,
-                          "mssp_ms": ms_ms, "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
+                          "bfs_gteps": bfs_edges / (bfs_ms / 1e3) / 1e9,
+                          "mssp_ms": ms_ms, "mssp_gteps": ms_edges / (ms_ms / 1e3) / 1e9,
+                          "mssp_sources_per_s": len(src) / (ms_ms / 1e3),
                           "cpu_reference_sources_per_s": len(src) / t_cpu, "cpu_threads": threads,
                           "cpu_dense_bfs_ms": 1e3 * t_dense, "rows_bit_exact": ok}), flush=True)
         g.close()
@@ -317,10 +321,12 @@ def run_suite(dawn, orc, steps, only=SUITE):
         fe = st.get("frontier_edges")
         print(json.dumps({"config": "grid2048: 2048^2 grid + 1% long-range, int32 w in [1,1000], seed 2",
                           "sssp_ms": sssp_ms, "sssp_steps": st["steps"],
+                          "sssp_gteps": st["edges_reached"] / (sssp_ms / 1e3) / 1e9,
                           "sssp_mode_steps": st["mode_steps"],
                           "sssp_relaxations_per_edge": fe / m if fe else None,
                           "sssp_roofline_frac": b_one / (sssp_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "mssp16_ms": ms_ms, "mssp16_steps": ms_steps,
+                          "mssp16_gteps": ms_st["edges_reached"] / (ms_ms / 1e3) / 1e9,
                           "mssp16_mode_steps": ms_st["mode_steps"], "forced_modes": by_policy,
                           "mssp16_roofline_frac": b_batch / (ms_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "sssp_x16_ms": singles_ms,

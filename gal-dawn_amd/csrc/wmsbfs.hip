@@ -413,12 +413,14 @@ __global__ void w_advance_kernel(WState* st, int32_t n, int32_t policy) {
 // stages 256 vertices x L lanes in LDS (one contiguous read), then each
 // wave writes its 64 vertices of every lane as one coalesced row segment.
 template <int L>
+// acc: [0..63] distance sums, [64..127] reached vertices, [128..191] sum of
+// out-degrees of reached vertices, per lane.
 __global__ void __launch_bounds__(kBlock) w_final_kernel(WArgs a, int32_t* out, int64_t stride,
-                                                         unsigned long long* sums) {
+                                                         unsigned long long* acc) {
   __shared__ int32_t s_tile[kBlock * L];
-  __shared__ unsigned long long s_sum[64];
+  __shared__ unsigned long long s_acc[3 * 64];
   const int nl = a.batch->nlanes;
-  if (threadIdx.x < 64) s_sum[threadIdx.x] = 0;
+  if (threadIdx.x < 3 * 64) s_acc[threadIdx.x] = 0;
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < a.n; b0 += (int64_t)gridDim.x * kBlock) {
@@ -429,19 +431,28 @@ __global__ void __launch_bounds__(kBlock) w_final_kernel(WArgs a, int32_t* out, 
     __syncthreads();
     const int64_t v = b0 + wv * kWave + lane;
     const bool ok = v < a.n;
+    const unsigned deg = ok ? (unsigned)(a.row_ptr[v + 1] - a.row_ptr[v]) : 0u;
     for (int bl = 0; bl < nl; bl++) {
       const int32_t d = ok ? s_tile[(wv * kWave + lane) * L + bl] : kInf;
       if (out && ok) out[(int64_t)bl * stride + v] = d;
-      const unsigned long long sm = wave_sum_u64(d != kInf ? (unsigned long long)d : 0ull);
-      if (lane == 0 && sm) atomicAdd(&s_sum[bl], sm);
+      const bool r = d != kInf;
+      const unsigned long long sm = wave_sum_u64(r ? (unsigned long long)d : 0ull);
+      const unsigned long long em = wave_sum_u64(r ? (unsigned long long)deg : 0ull);
+      const unsigned long long rc = (unsigned long long)__popcll(__ballot(r));
+      if (lane == 0) {
+        if (sm) atomicAdd(&s_acc[bl], sm);
+        if (rc) atomicAdd(&s_acc[64 + bl], rc);
+        if (em) atomicAdd(&s_acc[128 + bl], em);
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x < nl && s_sum[threadIdx.x]) atomicAdd(&sums[threadIdx.x], s_sum[threadIdx.x]);
+  for (int i = threadIdx.x; i < 3 * 64; i += kBlock)
+    if ((i & 63) < nl && s_acc[i]) atomicAdd(&acc[i], s_acc[i]);
 }
 
 __global__ void w_zero_kernel(unsigned long long* p, int n) {
-  if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
 
 }  // namespace
@@ -494,7 +505,7 @@ int ensure_wms(dawn_graph* g, int L) {
   DAWN_TRY(ws->q.alloc((size_t)2 * g->n));
   DAWN_TRY(ws->st.alloc(1));
   DAWN_TRY(ws->batch.alloc(1));
-  DAWN_TRY(ws->sums.alloc(64));
+  DAWN_TRY(ws->sums.alloc(3 * 64));
   if (!ws->h_state) DAWN_HIP(hipHostMalloc((void**)&ws->h_state, sizeof(WState)));
   if (!ws->h_batch) DAWN_HIP(hipHostMalloc((void**)&ws->h_batch, sizeof(WBatch)));
   for (hipEvent_t* e : {&ws->ev0, &ws->ev1})
@@ -561,7 +572,7 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
   }
 
   float total_ms = 0.f;
-  std::vector<unsigned long long> h_sums(64);
+  std::vector<unsigned long long> h_sums(3 * 64);
   for (int32_t b0 = 0; b0 < S; b0 += L) {
     const int32_t lanes = std::min<int32_t>(L, S - b0);
     WBatch& hb = *ws->h_batch;
@@ -587,11 +598,11 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
       if (launched > (int64_t)g->n + 2 * chunk) return DAWN_ERR_HIP;
     }
     int32_t* out = dist ? (dist_dev ? dist + (int64_t)b0 * g->n : ws->rows.p) : nullptr;
-    w_zero_kernel<<<1, 64, 0, s>>>(ws->sums.p, 64);
+    w_zero_kernel<<<1, 64, 0, s>>>(ws->sums.p, 3 * 64);
     w_final_kernel<L><<<fin_grid, kBlock, 0, s>>>(a, out, g->n, ws->sums.p);
     DAWN_HIP(hipGetLastError());
     DAWN_HIP(hipEventRecord(ws->ev1, s));
-    DAWN_HIP(hipMemcpyAsync(h_sums.data(), ws->sums.p, sizeof(unsigned long long) * 64,
+    DAWN_HIP(hipMemcpyAsync(h_sums.data(), ws->sums.p, sizeof(unsigned long long) * 3 * 64,
                             hipMemcpyDeviceToHost, s));
     DAWN_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
@@ -604,6 +615,10 @@ int run_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, int64
       if (sums) sums[b0 + i] = (int64_t)h_sums[i];
     ws->last_steps = ws->h_state->step;
     if (stats) {
+      for (int i = 0; i < lanes; i++) {
+        stats->vertices_reached += (int64_t)h_sums[64 + i];
+        stats->edges_reached += (int64_t)h_sums[128 + i];
+      }
       stats->batches += 1;
       stats->steps += ws->h_state->step;
       stats->mode_steps[DAWN_MODE_PUSH] += ws->h_state->step - ws->h_state->pull_steps;

@@ -310,11 +310,16 @@ def test_weighted_batched_mssp_exact(orc, count):
     if count > 5:
         srcs[5] = srcs[2]
     odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, h.weight, srcs)
+    deg = h.degrees().astype(np.int64)
+    reached = odist != INF
     for p in ("fixed", "push", "pull"):
         dist, sums, st = g.mssp(srcs, policy=p, stats=True)
         assert np.array_equal(dist, odist), p
         assert np.array_equal(sums, osums), p
         assert st["batches"] == -(-count // 64)
+        # Reached vertices and the out-degrees of reached vertices (GTEPS).
+        assert st["vertices_reached"] == int(reached.sum())
+        assert st["edges_reached"] == int((reached * deg[None, :]).sum())
         if p == "pull":
             assert st["mode_steps"]["push"] == 0 and st["mode_steps"]["pull"] > 0
         if p == "push":
