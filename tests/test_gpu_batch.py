@@ -72,13 +72,14 @@ def test_batch_split_tasks_duplicates_isolated(orc):
 
 def test_batch_weighted_multigraph_and_directed(orc):
     """Weighted tasks (near-far relaxation) on a multigraph whose sources
-    list duplicate edges with other weights (the last listed seeds) and on a
-    directed graph with weights spanning three orders of magnitude."""
+    list duplicate edges with other weights (the last listed seeds), with
+    zero-weight edges, and on a directed graph with weights spanning three
+    orders of magnitude."""
     rng = np.random.default_rng(17)
     n = 3000
     u = rng.integers(0, n, 20000).astype(np.int32)
     v = rng.integers(0, n, 20000).astype(np.int32)
-    w = rng.integers(1, 1000, 20000).astype(np.int32)
+    w = rng.integers(0, 1000, 20000).astype(np.int32)  # zero weights included
     at = u < 40
     u = np.concatenate([u, u[at], u[at]])
     v = np.concatenate([v, v[at], v[at]])

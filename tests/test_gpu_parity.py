@@ -581,3 +581,25 @@ def test_minplus_sums_beyond_int32_are_unreachable(orc):
     assert list(sums) == [int(want[s][want[s] != INF].astype(np.int64).sum()) for s in range(6)]
     _, csums, _ = g.closeness(srcs)
     assert np.array_equal(csums, sums)
+
+
+def test_zero_weight_edges_all_engines(orc):
+    """Zero-weight edges (allowed: weights are non-negative): single-source
+    min-plus under every mode and the batched engine under push, pull and
+    the rule agree with Dijkstra."""
+    rng = np.random.default_rng(23)
+    n = 4000
+    u = rng.integers(0, n, 24000).astype(np.int32)
+    v = rng.integers(0, n, 24000).astype(np.int32)
+    w = rng.integers(0, 5, 24000).astype(np.int32)  # many zeros
+    h = dawn.HostGraph.from_edges(n, u, v, w, symmetrize=True)
+    assert (h.weight == 0).sum() > 1000
+    g = dawn.Graph(h)
+    srcs = [int(s) for s in dawn.gen_sources(n, 12, 23) if h.degrees()[s] > 0]
+    want = np.stack([orc.dijkstra(h.row_ptr, h.col, h.weight, s) for s in srcs])
+    for i, s in enumerate(srcs[:4]):
+        for p in ["fixed", "push", "pull", "bitmap"]:
+            assert np.array_equal(g.sssp(s, policy=p), want[i]), (s, p)
+    for p in ["fixed", "push", "pull"]:
+        dist, _ = g.mssp(np.array(srcs, np.int32), policy=p)
+        assert np.array_equal(dist, want), p
