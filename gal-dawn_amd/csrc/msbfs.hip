@@ -1361,8 +1361,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fin
 ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
   constexpr int KS = level_slices(W);
   static_assert((1 << level_slices(W)) >= hist_slots(W), "window levels fit the slices");
-  constexpr int TPB = W < 4 ? 4 / W : 1;  // vertex tiles per workgroup
-  constexpr int BPT = W > 4 ? W / 4 : 1;  // workgroups per tile (grid is a multiple)
   __shared__ unsigned long long s_sum[64 * W], s_reach[64 * W], s_edges[64 * W];
   int lo = lo_arg, hi = hi_arg;
   if (from_state) {
@@ -1388,14 +1386,21 @@ ms_final_kernel(MsArgs a, MsOut o, int from_state, int lo_arg, int hi_arg) {
   while ((1 << K) < hi - lo) K++;
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  // Lane word of this wave, and its tile: W <= 4: the workgroup's four
-  // waves cover 4/W tiles; W = 8: two workgroups share a tile.
-  // W > 4: the grid is a multiple of kXcds * BPT; workgroup x runs on XCD
-  // x % kXcds, and r = x / kXcds counts that XCD's workgroups, BPT per tile.
+  // Lane word of this wave, and its tile, over the G words that hold the
+  // batch's lanes (G = the power of two >= its live words, at most W: a
+  // partly filled last batch spreads its few words over every wave instead
+  // of leaving the waves of empty words idle).  G <= 4: the workgroup's four
+  // waves cover 4/G tiles; G = 8: two workgroups share a tile.  G > 4: the
+  // grid is a multiple of kXcds * G/4; workgroup x runs on XCD x % kXcds,
+  // and r = x / kXcds counts that XCD's workgroups, G/4 per tile.
+  int G = 1;
+  while (G < W && 64 * G < nl) G <<= 1;
+  const int BPT = G > 4 ? G / 4 : 1;  // workgroups per tile
+  const int TPB = G < 4 ? 4 / G : 1;  // vertex tiles per workgroup
   const unsigned xr = blockIdx.x / kXcds;
-  const int wi = W > 4 ? (int)(xr % BPT) * 4 + wv : wv % W;
-  const int tile = W > 4 ? 0 : wv / W;
-  const int64_t tile0 = W > 4 ? (int64_t)(xr / BPT) * kXcds + blockIdx.x % kXcds
+  const int wi = G > 4 ? (int)(xr % BPT) * 4 + wv : wv % G;
+  const int tile = G > 4 ? 0 : wv / G;
+  const int64_t tile0 = G > 4 ? (int64_t)(xr / BPT) * kXcds + blockIdx.x % kXcds
                               : (int64_t)blockIdx.x * TPB + tile;
   const int rows = min(64, nl - 64 * wi);  // lanes of this word in the batch
   // The first window writes every (lane, vertex): 16-byte row stores when
