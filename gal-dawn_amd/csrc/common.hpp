@@ -133,6 +133,7 @@ struct dawn_graph {
   std::mutex lock;                // one traversal per graph at a time
   dawn::MsWorkspace* ms = nullptr;
   dawn::MsWorkspace* ms2 = nullptr;   // second batch in flight (stream2)
+  dawn::MsWorkspace* ms_tail = nullptr;  // a partly filled last batch, fewer lane words
   hipStream_t stream2 = nullptr;
   dawn::SsWorkspace* ss = nullptr;
   dawn::WmsWorkspace* wms = nullptr;

@@ -224,6 +224,7 @@ int dawn_graph_destroy(dawn_graph* g) {
   if (g->stream2) (void)hipStreamSynchronize(g->stream2);
   ms_workspace_free(g->ms);
   ms_workspace_free(g->ms2);
+  ms_workspace_free(g->ms_tail);
   ss_workspace_free(g->ss);
   wms_workspace_free(g->wms);
   ms_view_free(g);

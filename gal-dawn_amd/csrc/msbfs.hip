@@ -181,7 +181,7 @@ void wms_invalidate_graphs(WmsWorkspace* ws);
 // heavy-chunk lists among them); the autotuner and workspace re-layouts
 // drop them.
 void invalidate_graphs(dawn_graph* g) {
-  for (MsWorkspace* ws : {g->ms, g->ms2}) {
+  for (MsWorkspace* ws : {g->ms, g->ms2, g->ms_tail}) {
     if (!ws) continue;
     for (auto& kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
     ws->graphs.clear();
@@ -1970,6 +1970,13 @@ int run_msbfs_slot(dawn_graph* g, MsWorkspace* ws, hipStream_t s, const int32_t*
 // batch's memory-bound distance pass overlaps the other's steps; if the
 // second workspace cannot be allocated, one stream runs them all.  Then
 // stats->device_ms is the wall time of the whole set of batches.
+//
+// A partly filled last batch that fits fewer lane words runs on records of
+// that width, in a workspace of its own (kept with the graph), on the
+// stream whose turn it is: a step gathers whole records whatever the
+// number of live lanes in them, so a 120-lane batch at 16 words costs about
+// as much as a full one.  If that workspace cannot be allocated, the last
+// batch runs at W.
 template <int W>
 int run_msbfs_w(dawn_graph* g, const int32_t* sources, const int32_t* rowidx, int32_t S,
                 const IsoRows* iso, int32_t* dist, int64_t* sums, int64_t* reached,
@@ -1996,9 +2003,55 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, const int32_t* rowidx, in
       two = false;
     }
   }
-  if (!two)
-    return run_msbfs_slot<W>(g, g->ms, g->stream, sources, rowidx, S, 0, 1, iso, dist, sums,
-                             reached, edges, policy, stats, rows);
+  const int32_t full = S / kB * kB;
+  const int32_t tail = S - full;
+  int Wt = 1;
+  while (Wt < W && 64 * Wt < tail) Wt <<= 1;
+  bool narrow = full > 0 && tail > 0 && Wt < W;
+  if (narrow && ensure_ms(g, g->ms_tail, Wt) != DAWN_OK) {
+    ms_workspace_free(g->ms_tail);
+    g->ms_tail = nullptr;
+    (void)hipGetLastError();
+    narrow = false;
+  }
+  std::vector<int32_t> tail_rows;
+  const int32_t* trow = nullptr;
+  if (narrow) {
+    if (rowidx) {
+      trow = rowidx + full;
+    } else {
+      tail_rows.resize(tail);
+      for (int32_t i = 0; i < tail; i++) tail_rows[i] = full + i;
+      trow = tail_rows.data();
+    }
+  }
+  const int32_t S_main = narrow ? full : S;
+  // The narrow last batch, on stream s (nothing to do unless there is one).
+  auto run_tail = [&](hipStream_t s, dawn_stats* st) -> int {
+    if (!narrow) return DAWN_OK;
+    const int32_t* ts = sources + full;
+    switch (Wt) {
+      case 1:
+        return run_msbfs_slot<1>(g, g->ms_tail, s, ts, trow, tail, 0, 1, nullptr, dist, sums,
+                                 reached, edges, policy, st, rows);
+      case 2:
+        return run_msbfs_slot<2>(g, g->ms_tail, s, ts, trow, tail, 0, 1, nullptr, dist, sums,
+                                 reached, edges, policy, st, rows);
+      case 4:
+        return run_msbfs_slot<4>(g, g->ms_tail, s, ts, trow, tail, 0, 1, nullptr, dist, sums,
+                                 reached, edges, policy, st, rows);
+      default:
+        return run_msbfs_slot<8>(g, g->ms_tail, s, ts, trow, tail, 0, 1, nullptr, dist, sums,
+                                 reached, edges, policy, st, rows);
+    }
+  };
+  // The narrow batch (batch full/kB) runs on the stream of its turn.
+  const int tail_turn = (int)((full / kB) % 2);
+  if (!two) {
+    DAWN_TRY(run_msbfs_slot<W>(g, g->ms, g->stream, sources, rowidx, S_main, 0, 1, iso, dist,
+                               sums, reached, edges, policy, stats, rows));
+    return run_tail(g->stream, stats);
+  }
   dawn_stats st1{};
   int rc1 = DAWN_OK;
   const auto t0 = std::chrono::steady_clock::now();
@@ -2009,12 +2062,14 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, const int32_t* rowidx, in
     }
     // The second stream runs the fewer batches: it also writes the rows of
     // the isolated sources.
-    rc1 = run_msbfs_slot<W>(g, g->ms2, g->stream2, sources, rowidx, S, 1, 2, iso, dist, sums,
+    rc1 = run_msbfs_slot<W>(g, g->ms2, g->stream2, sources, rowidx, S_main, 1, 2, iso, dist, sums,
                             reached, edges, policy, stats ? &st1 : nullptr, rows);
+    if (rc1 == DAWN_OK && tail_turn == 1) rc1 = run_tail(g->stream2, stats ? &st1 : nullptr);
   });
   dawn_stats st0{};
-  const int rc0 = run_msbfs_slot<W>(g, g->ms, g->stream, sources, rowidx, S, 0, 2, nullptr, dist,
-                                    sums, reached, edges, policy, stats ? &st0 : nullptr, rows);
+  int rc0 = run_msbfs_slot<W>(g, g->ms, g->stream, sources, rowidx, S_main, 0, 2, nullptr, dist,
+                              sums, reached, edges, policy, stats ? &st0 : nullptr, rows);
+  if (rc0 == DAWN_OK && tail_turn == 0) rc0 = run_tail(g->stream, stats ? &st0 : nullptr);
   second.join();
   const double wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2117,13 +2172,15 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
 // seed, decide, expand push, expand pull, expand bitmap, finalize, final).
 // levels_out: number of levels of each batch of 64·W sources, -1 past the
 // last batch (one entry per 64 sources).
+// Adds the kernel times of the batches of `sources` at W lane words, run in
+// workspace `slot`, to kernel_ms / launches; levels_out[i]: levels of batch i.
 template <int W>
-int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
-                     double* kernel_ms, int64_t* launches, int32_t* levels_out) {
+int run_ms_profile_w(dawn_graph* g, MsWorkspace*& slot, const int32_t* sources, int32_t S,
+                     int32_t policy, double* kernel_ms, int64_t* launches, int32_t* levels_out) {
   constexpr int kB = 64 * W;
-  DAWN_TRY(ensure_ms(g, g->ms, W));
+  DAWN_TRY(ensure_ms(g, slot, W));
   DAWN_TRY(ms_view_ensure(g));
-  MsWorkspace* ws = g->ms;
+  MsWorkspace* ws = slot;
   DAWN_TRY(ws->distbuf.alloc((size_t)kB * g->n));
   hipStream_t s = g->stream;
   MsArgs a = make_args(g, ws);
@@ -2134,12 +2191,6 @@ int run_ms_profile_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t p
   const unsigned fgrid = ms_final_grid(g, W);
   unsigned long long* ls = ws->lane_stats.p;
   MsOut o{ls, ls + kMaxLanes, ls + 2 * kMaxLanes, 1};
-  for (int i = 0; i < 8; i++) {
-    kernel_ms[i] = 0;
-    launches[i] = 0;
-  }
-  if (levels_out)
-    for (int32_t i = 0; i < (S + 63) / 64; i++) levels_out[i] = -1;
   auto timed = [&](int slot, auto&& launch) -> int {
     DAWN_HIP(hipEventRecord(ws->e_a, s));
     launch();
@@ -2266,6 +2317,8 @@ int run_msbfs(dawn_graph* g, const int32_t* sources, int32_t S, int32_t* dist, i
       g->ms = nullptr;
       ms_workspace_free(g->ms2);
       g->ms2 = nullptr;
+      ms_workspace_free(g->ms_tail);
+      g->ms_tail = nullptr;
       (void)hipGetLastError();
       continue;
     }
@@ -2338,17 +2391,28 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   S = (int32_t)live.size();
   if (levels_out)
     for (int32_t i = 0; i < (S_all + 63) / 64; i++) levels_out[i] = -1;
-  if (S == 0) {
-    for (int i = 0; i < 8; i++) kernel_ms[i] = 0, launches[i] = 0;
-    return DAWN_OK;
-  }
-  switch (words_for(g, S)) {
-    case 1: return run_ms_profile_w<1>(g, sources, S, policy, kernel_ms, launches, levels_out);
-    case 2: return run_ms_profile_w<2>(g, sources, S, policy, kernel_ms, launches, levels_out);
-    case 4: return run_ms_profile_w<4>(g, sources, S, policy, kernel_ms, launches, levels_out);
-    case 8: return run_ms_profile_w<8>(g, sources, S, policy, kernel_ms, launches, levels_out);
-    default: return run_ms_profile_w<16>(g, sources, S, policy, kernel_ms, launches, levels_out);
-  }
+  for (int i = 0; i < 8; i++) kernel_ms[i] = 0, launches[i] = 0;
+  if (S == 0) return DAWN_OK;
+  // The batches as the traversal runs them: full ones at W lane words, a
+  // partly filled last one at the fewest words holding its lanes.
+  const int W = words_for(g, S);
+  const int32_t full = S / (64 * W) * (64 * W);
+  int Wt = 1;
+  while (Wt < W && 64 * Wt < S - full) Wt <<= 1;
+  const int32_t S_main = full > 0 && Wt < W ? full : S;
+  auto run = [&](int w, MsWorkspace*& slot, const int32_t* src, int32_t cnt, int32_t* lv) -> int {
+    switch (w) {
+      case 1: return run_ms_profile_w<1>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
+      case 2: return run_ms_profile_w<2>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
+      case 4: return run_ms_profile_w<4>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
+      case 8: return run_ms_profile_w<8>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
+      default: return run_ms_profile_w<16>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
+    }
+  };
+  DAWN_TRY(run(W, g->ms, sources, S_main, levels_out));
+  if (S_main == S) return DAWN_OK;
+  return run(Wt, g->ms_tail, sources + full, S - full,
+             levels_out ? levels_out + full / (64 * W) : nullptr);
 }
 
 }  // namespace dawn

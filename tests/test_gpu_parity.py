@@ -434,6 +434,38 @@ def test_wide_batches_hub_view_exact(orc, directed):
         assert st["batches"] == 2 and st["skipped_sources"] == 9
 
 
+@pytest.mark.parametrize("count", [1024 + 40, 1024 + 100, 2048 + 200, 3072 + 300, 1024 + 700])
+def test_narrow_last_batch_exact(orc, count):
+    """A partly filled last batch runs at the fewest lane words holding its
+    lanes (40 lanes: 1 word, 100: 2, 200: 4, 300: 8; 700 needs all 16), on
+    the stream of its turn (after 1 or 3 full batches the second stream,
+    after 2 the first): host rows, device rows, sums-only closeness and the
+    profiling pass's batches equal the oracle's."""
+    log(f"case: RMAT scale 12 ef 8 seed 31, {count} live sources, lane_words=16")
+    h = dawn.HostGraph.rmat(12, 8, 31)
+    g = dawn.Graph(h)
+    g.set_knobs(lane_words=16, hub_order=1)
+    deg = h.degrees()
+    rng = np.random.default_rng(count)
+    srcs = rng.choice(np.flatnonzero(deg > 0), count).astype(np.int32)  # duplicates too
+    srcs = np.insert(srcs, [5, count // 2], np.flatnonzero(deg == 0)[:2]).astype(np.int32)
+    odist, osums, skipped, _ = orc.mssp(h.row_ptr, h.col, None, srcs)
+    assert int(skipped.sum()) == 2
+    for p in ("fixed", "learned", "pull", "push"):
+        dist, sums, st = g.mssp(srcs, policy=p, stats=True)
+        assert np.array_equal(dist, odist), p
+        assert np.array_equal(sums, osums), p
+        assert st["batches"] == -(-count // 1024) and st["skipped_sources"] == 2
+    dev = dawn.DeviceArray((len(srcs), h.n), np.int32).fill_bytes(0x5A)
+    _, sums = g.mssp(srcs, policy="learned", out=dev)
+    assert np.array_equal(dev.download(), odist) and np.array_equal(sums, osums)
+    dev.close()
+    _, csums, _ = g.closeness(srcs, policy="learned")
+    assert np.array_equal(csums, osums)
+    _, lv = g.mssp_profile(srcs, policy="learned")
+    assert len(lv) == -(-count // 1024) and (lv > 0).all()
+
+
 def _directed_csr(n, u, v):
     order = np.lexsort((v, u))
     u, v = u[order], v[order]
