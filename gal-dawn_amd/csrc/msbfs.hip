@@ -1623,6 +1623,24 @@ int words_for(const dawn_graph* g, int32_t S) {
   return w;
 }
 
+// The batches of a traversal of S live sources: full ones of 64·W lanes
+// (the first S_main sources) and, when the last batch is partly filled and
+// its lanes fit fewer words, that batch at Wt words (run_msbfs_w makes the
+// same split, and runs the last batch at W if its workspace does not fit).
+struct BatchSplit {
+  int W = 1, Wt = 1;
+  int32_t S_main = 0;
+};
+BatchSplit split_batches(const dawn_graph* g, int32_t S) {
+  BatchSplit b;
+  b.W = words_for(g, S);
+  const int32_t full = S / (64 * b.W) * (64 * b.W);
+  b.Wt = 1;
+  while (b.Wt < b.W && 64 * b.Wt < S - full) b.Wt <<= 1;
+  b.S_main = full > 0 && b.Wt < b.W ? full : S;
+  return b;
+}
+
 int ensure_ms(dawn_graph* g, MsWorkspace*& slot, int W) {
   if (!slot) slot = new MsWorkspace();
   MsWorkspace* ws = slot;
@@ -2102,14 +2120,16 @@ int run_msbfs_w(dawn_graph* g, const int32_t* sources, const int32_t* rowidx, in
 // Per-step mode timings from identical state (scheduler training data).
 // Each step: decide (features), snapshot, then every mode is run `repeats`
 // times from the snapshot and timed with HIP events; the minimum is kept.
+// The batches of `sources` at W lane words run in workspace `slot`; rows
+// are written from index 0 of the given arrays.
 template <int W>
-int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repeats,
-                    float* features, float* times_ms, int32_t* chosen, int32_t max_rows,
-                    int32_t* rows_out, int32_t policy) {
+int run_ms_record_w(dawn_graph* g, MsWorkspace*& slot, const int32_t* sources, int32_t S,
+                    int32_t repeats, float* features, float* times_ms, int32_t* chosen,
+                    int32_t max_rows, int32_t* rows_out, int32_t policy) {
   constexpr int kB = 64 * W;
-  DAWN_TRY(ensure_ms(g, g->ms, W));
+  DAWN_TRY(ensure_ms(g, slot, W));
   DAWN_TRY(ms_view_ensure(g));
-  MsWorkspace* ws = g->ms;
+  MsWorkspace* ws = slot;
   const size_t words = (size_t)ws->npad * W;
   DAWN_TRY(ws->seen_bak.alloc(words));
   DAWN_TRY(ws->done_bak.alloc((size_t)(ws->npad / 32)));
@@ -2179,12 +2199,10 @@ int run_ms_record_w(dawn_graph* g, const int32_t* sources, int32_t S, int32_t re
 
 // Same kernels as run_msbfs, launched one by one with HIP events around each
 // launch on the graph's stream, writing distance rows and statistics like a
-// timed traversal.  kernel_ms / launches are indexed by DAWN_PROF_* (fill,
-// seed, decide, expand push, expand pull, expand bitmap, finalize, final).
-// levels_out: number of levels of each batch of 64·W sources, -1 past the
-// last batch (one entry per 64 sources).
-// Adds the kernel times of the batches of `sources` at W lane words, run in
-// workspace `slot`, to kernel_ms / launches; levels_out[i]: levels of batch i.
+// timed traversal: the batches of `sources` at W lane words, in workspace
+// `slot`.  Adds to kernel_ms / launches, indexed by DAWN_PROF_* (fill, seed,
+// decide, expand push, expand pull, expand bitmap, finalize, final);
+// levels_out[i]: the number of levels of batch i.
 template <int W>
 int run_ms_profile_w(dawn_graph* g, MsWorkspace*& slot, const int32_t* sources, int32_t S,
                      int32_t policy, double* kernel_ms, int64_t* launches, int32_t* levels_out) {
@@ -2369,23 +2387,29 @@ int run_ms_record(dawn_graph* g, const int32_t* sources, int32_t S, int32_t repe
     if (rows_out) *rows_out = 0;
     return DAWN_OK;
   }
-  switch (words_for(g, S)) {
-    case 1:
-      return run_ms_record_w<1>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
-                                rows_out, policy);
-    case 2:
-      return run_ms_record_w<2>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
-                                rows_out, policy);
-    case 4:
-      return run_ms_record_w<4>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
-                                rows_out, policy);
-    case 8:
-      return run_ms_record_w<8>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
-                                rows_out, policy);
-    default:
-      return run_ms_record_w<16>(g, sources, S, repeats, features, times_ms, chosen, max_rows,
-                                 rows_out, policy);
-  }
+  // The batches as the traversal runs them (full ones, then a partly
+  // filled last one at its own width); the last batch's rows follow.
+  const BatchSplit bs = split_batches(g, S);
+  auto run = [&](int w, MsWorkspace*& slot, const int32_t* src, int32_t cnt, int32_t off,
+                 int32_t* got) -> int {
+    float* f = features + (size_t)off * 8;
+    float* t = times_ms + (size_t)off * DAWN_NUM_MODES;
+    int32_t* c = chosen ? chosen + off : nullptr;
+    const int32_t mr = max_rows - off;
+    switch (w) {
+      case 1: return run_ms_record_w<1>(g, slot, src, cnt, repeats, f, t, c, mr, got, policy);
+      case 2: return run_ms_record_w<2>(g, slot, src, cnt, repeats, f, t, c, mr, got, policy);
+      case 4: return run_ms_record_w<4>(g, slot, src, cnt, repeats, f, t, c, mr, got, policy);
+      case 8: return run_ms_record_w<8>(g, slot, src, cnt, repeats, f, t, c, mr, got, policy);
+      default: return run_ms_record_w<16>(g, slot, src, cnt, repeats, f, t, c, mr, got, policy);
+    }
+  };
+  int32_t r0 = 0, r1 = 0;
+  DAWN_TRY(run(bs.W, g->ms, sources, bs.S_main, 0, &r0));
+  if (bs.S_main < S && r0 < max_rows)
+    DAWN_TRY(run(bs.Wt, g->ms_tail, sources + bs.S_main, S - bs.S_main, r0, &r1));
+  if (rows_out) *rows_out = r0 + r1;
+  return DAWN_OK;
 }
 
 int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t policy,
@@ -2406,11 +2430,7 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
   if (S == 0) return DAWN_OK;
   // The batches as the traversal runs them: full ones at W lane words, a
   // partly filled last one at the fewest words holding its lanes.
-  const int W = words_for(g, S);
-  const int32_t full = S / (64 * W) * (64 * W);
-  int Wt = 1;
-  while (Wt < W && 64 * Wt < S - full) Wt <<= 1;
-  const int32_t S_main = full > 0 && Wt < W ? full : S;
+  const BatchSplit bs = split_batches(g, S);
   auto run = [&](int w, MsWorkspace*& slot, const int32_t* src, int32_t cnt, int32_t* lv) -> int {
     switch (w) {
       case 1: return run_ms_profile_w<1>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
@@ -2420,10 +2440,10 @@ int run_ms_profile(dawn_graph* g, const int32_t* sources, int32_t S, int32_t pol
       default: return run_ms_profile_w<16>(g, slot, src, cnt, policy, kernel_ms, launches, lv);
     }
   };
-  DAWN_TRY(run(W, g->ms, sources, S_main, levels_out));
-  if (S_main == S) return DAWN_OK;
-  return run(Wt, g->ms_tail, sources + full, S - full,
-             levels_out ? levels_out + full / (64 * W) : nullptr);
+  DAWN_TRY(run(bs.W, g->ms, sources, bs.S_main, levels_out));
+  if (bs.S_main == S) return DAWN_OK;
+  return run(bs.Wt, g->ms_tail, sources + bs.S_main, S - bs.S_main,
+             levels_out ? levels_out + bs.S_main / (64 * bs.W) : nullptr);
 }
 
 }  // namespace dawn

@@ -464,6 +464,11 @@ def test_narrow_last_batch_exact(orc, count):
     assert np.array_equal(csums, osums)
     _, lv = g.mssp_profile(srcs, policy="learned")
     assert len(lv) == -(-count // 1024) and (lv > 0).all()
+    # The per-step mode timings record the same batches: one row per step
+    # of the fixed schedule's traversal.
+    _, _, st = g.mssp(srcs, policy="fixed", want_dist=False, stats=True)
+    modes, _, times = g.sched_trace(srcs, policy="fixed")
+    assert len(modes) == st["steps"] and (times > 0).all()
 
 
 def _directed_csr(n, u, v):
