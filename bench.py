@@ -514,7 +514,7 @@ def main():
         # starting from the engine's defaults, not from the cached knobs.
         t0 = time.time()
         g.set_knobs(**DEFAULT_KNOBS)
-        dawn.tune(g)
+        dawn.tune(g, verbose=True)
         tune_s = time.time() - t0
         if args.tune_out:
             dawn.tune_save(args.tune_out)

@@ -153,7 +153,10 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
                    "%.3f ms\n",
                    class_key(g).c_str(), k.dense_blocks, k.heavy_degree, k.pull_chunk, k.ss_chunk,
                    k.pull_variant, k.lane_words, k.hub_order, ms);
-    if (ms < best_ms) {
+    // A candidate must beat the best by 0.5%: repeated passes with the same
+    // knobs vary by up to 0.4% (RMAT-24), and a pick made on noise is not
+    // kept.
+    if (ms < 0.995 * best_ms) {
       best_ms = ms;
       best = k;
     }
@@ -167,35 +170,39 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
     }
   } else {
     // The same sample sources at every width; the width first, since the
-    // best vertex order and launch shapes depend on it.
+    // best vertex order and launch shapes depend on it.  Then two rounds of
+    // coordinate descent over the other knobs: the second re-checks each
+    // one under the others' values from the first.
     for (int lw : {1, 2, 4, 8, 16}) {
       TuneKnobs k = best;
       k.lane_words = lw;
       DAWN_TRY(consider(k));
     }
-    for (int ho : {0, 1}) {
-      TuneKnobs k = best;
-      k.hub_order = ho;
-      DAWN_TRY(consider(k));
-    }
-    for (int pv : {0, 1, 2, 3}) {
-      TuneKnobs k = best;
-      k.pull_variant = pv;
-      DAWN_TRY(consider(k));
-    }
-    for (int b : {1024, 2048, 4096, 8192, 16384}) {
-      TuneKnobs k = best;
-      k.dense_blocks = b;
-      DAWN_TRY(consider(k));
-    }
-    const int pairs[][2] = {{16, 256}, {32, 128}, {32, 256}, {32, 512}, {32, 1024},
-                            {48, 512}, {48, 1024}, {64, 256}, {64, 1024}};
-    const TuneKnobs base = best;
-    for (auto& p : pairs) {
-      TuneKnobs k = base;
-      k.heavy_degree = p[0];
-      k.pull_chunk = p[1];
-      DAWN_TRY(consider(k));
+    for (int round = 0; round < 2; round++) {
+      for (int ho : {0, 1}) {
+        TuneKnobs k = best;
+        k.hub_order = ho;
+        DAWN_TRY(consider(k));
+      }
+      for (int pv : {0, 1, 2, 3}) {
+        TuneKnobs k = best;
+        k.pull_variant = pv;
+        DAWN_TRY(consider(k));
+      }
+      for (int b : {1024, 2048, 4096, 8192, 16384}) {
+        TuneKnobs k = best;
+        k.dense_blocks = b;
+        DAWN_TRY(consider(k));
+      }
+      const int pairs[][2] = {{16, 256}, {32, 128}, {32, 256}, {32, 512}, {32, 1024},
+                              {48, 512}, {48, 1024}, {64, 256}, {64, 1024}};
+      const TuneKnobs base = best;
+      for (auto& p : pairs) {
+        TuneKnobs k = base;
+        k.heavy_degree = p[0];
+        k.pull_chunk = p[1];
+        DAWN_TRY(consider(k));
+      }
     }
   }
   DAWN_TRY(apply(g, best));
