@@ -62,8 +62,10 @@ int sample_ms(dawn_graph* g, const std::vector<int32_t>& src, double* ms) {
         DAWN_TRY(run_single_source(g, 1, s, nullptr, DAWN_POLICY_FIXED, &st, -1, nullptr, nullptr,
                                    0, nullptr));
     } else {
+      // The learned schedule where a network is installed (the fixed rule
+      // otherwise), as traversals run by default.
       DAWN_TRY(run_msbfs(g, src.data(), (int32_t)src.size(), nullptr, nullptr, nullptr, nullptr,
-                         DAWN_POLICY_FIXED, &st, true));
+                         DAWN_POLICY_LEARNED, &st, true));
     }
     if (rep > 0) best = std::min(best, st.device_ms);
   }
@@ -114,15 +116,26 @@ int dawn_tune(const dawn_graph* gc, int32_t verbose) {
   dawn_graph* g = const_cast<dawn_graph*>(gc);
   if (!g || g->n <= 0) return DAWN_ERR_INVALID_ARG;
   // Sample sources.  Weighted: two spread vertices with edges (single-source
-  // traversals).  Unweighted: 4096 vertices spread evenly over the range,
-  // isolated ones included, as uniform random sources are.
+  // traversals).  Unweighted: 4096 uniform pseudo-random vertices (a fixed
+  // splitmix64 sequence), isolated ones included, as uniform random sources
+  // are.  Evenly spaced ids would not do: in a recursive generator such as
+  // RMAT, ids that are multiples of a power of two are the high-degree
+  // corner of the matrix.
   std::vector<int32_t> src;
   if (g->weighted) {
     for (int32_t v = 0; v < g->n && (int)src.size() < 2; v += std::max(1, g->n / 8))
       if (g->h_row_ptr[v + 1] > g->h_row_ptr[v]) src.push_back(v);
   } else {
     const int64_t want = 4096;
-    for (int64_t i = 0; i < want; i++) src.push_back((int32_t)(i * g->n / want));
+    uint64_t x = 0x5DEECE66Dull;
+    for (int64_t i = 0; i < want; i++) {
+      x += 0x9E3779B97F4A7C15ull;
+      uint64_t z = x;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      src.push_back((int32_t)(z % (uint64_t)g->n));
+    }
     bool any = false;
     for (int32_t v : src) any = any || g->h_row_ptr[v + 1] > g->h_row_ptr[v];
     for (int32_t v = 0; v < g->n && !any; v++)  // a vertex with edges, so that something runs
