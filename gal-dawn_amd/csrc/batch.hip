@@ -127,18 +127,40 @@ __device__ __forceinline__ int lds_append(unsigned* s_count, bool pred) {
   return pred ? (int)(base + (unsigned)__popcll(m & lanes_below_mask())) : -1;
 }
 
-template <int BS>
+// atomicOr on a lane word of type LT; sub-word types go through the
+// aligned 32-bit word that holds them.  Returns the old value of *p.
+template <typename LT>
+__device__ __forceinline__ LT atomic_or_lanes(LT* p, LT v) {
+  if constexpr (sizeof(LT) == 8) {
+    return (LT)atomicOr((unsigned long long*)p, (unsigned long long)v);
+  } else if constexpr (sizeof(LT) == 4) {
+    return (LT)atomicOr((unsigned*)p, (unsigned)v);
+  } else {
+    const uintptr_t ad = (uintptr_t)p;
+    const int sh = (int)(ad & 3) * 8;
+    const unsigned old = atomicOr((unsigned*)(ad & ~(uintptr_t)3), (unsigned)v << sh);
+    return (LT)(old >> sh);
+  }
+}
+
+// LT: the narrowest unsigned type holding the task's lanes (8 sources per
+// graph: one byte per vertex).  The per-vertex state (seen, next, queue
+// words) is probed and OR-ed at random per scanned edge, so its footprint
+// decides how much of it the caches hold: at 8 bytes per vertex the
+// workgroups' state spilled out of L2 and the Infinity Cache, and the
+// kernel moved about 100 GB for 512 graphs of 14 M vertices.
+template <int BS, typename LT>
 __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const BatchTask& T,
-                               uint64_t* seen, uint64_t* next, int32_t* qv, uint64_t* qw,
+                               LT* seen, LT* next, int32_t* qv, LT* qw,
                                int32_t* touched, int* s_off, int* s_beg, uint64_t* s_f,
                                int* s_scan, unsigned* s_cnt) {
   const int32_t n = G.n;
   const int64_t mn = a.max_n;
   // Lane b of the task is source lane0 + b of the graph: row drow + b * n.
   int32_t* drow = a.dist + G.dist_off + (int64_t)T.lane0 * n;
-  const uint64_t lanemask = T.lanes >= 64 ? ~0ull : ((1ull << T.lanes) - 1);
+  const LT lanemask = (LT)(T.lanes >= 64 ? ~0ull : ((1ull << T.lanes) - 1));
   for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
-    seen[v] = ~lanemask;
+    seen[v] = (LT)~lanemask;
     next[v] = 0;
   }
   __syncthreads();
@@ -146,7 +168,7 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
     s_cnt[0] = 0;  // |Q_0|
     for (int i = 0; i < T.lanes; i++) {
       const int32_t s = a.sources[(int64_t)T.graph * a.spg + T.lane0 + i];
-      const uint64_t bit = 1ull << i;
+      const LT bit = (LT)(1ull << i);
       if ((seen[s] & lanemask) == 0) {
         qv[s_cnt[0]] = s;
         qw[s_cnt[0]] = 0;
@@ -164,9 +186,9 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
   int level = 0;
   while (qn > 0) {
     const int32_t* qv_in = qv + (int64_t)par * mn;
-    const uint64_t* qw_in = qw + (int64_t)par * mn;
+    const LT* qw_in = qw + (int64_t)par * mn;
     int32_t* qv_out = qv + (int64_t)(par ^ 1) * mn;
-    uint64_t* qw_out = qw + (int64_t)(par ^ 1) * mn;
+    LT* qw_out = qw + (int64_t)(par ^ 1) * mn;
     if (threadIdx.x == 0) s_cnt[1] = 0;  // |touched|
     __syncthreads();
     // Expand Q_level with workgroup edge balancing (LDS degree scan).
@@ -193,8 +215,8 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
         if (e < total) {
           const int j = owner<BS>(s_off, e);
           w = G.col[s_beg[j] + (e - s_off[j])];
-          const uint64_t nv = s_f[j] & ~seen[w];
-          if (nv) first = atomicOr((unsigned long long*)&next[w], (unsigned long long)nv) == 0ull;
+          const LT nv = (LT)(s_f[j] & ~(uint64_t)seen[w]);
+          if (nv) first = atomic_or_lanes<LT>(&next[w], nv) == 0;
         }
         const int slot = lds_append(&s_cnt[1], first);
         if (first) touched[slot] = w;
@@ -210,16 +232,16 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
       const unsigned t = t0 + threadIdx.x;
       bool isf = false;
       int32_t w = 0;
-      uint64_t nw = 0;
+      LT nw = 0;
       if (t < tn) {
         w = touched[t];
-        const uint64_t nx = next[w];
+        const LT nx = next[w];
         next[w] = 0;
-        nw = nx & ~seen[w];
+        nw = (LT)(nx & ~seen[w]);
         if (nw) {
-          seen[w] |= nw;
+          seen[w] = (LT)(seen[w] | nw);
           isf = true;
-          uint64_t bits = nw;
+          uint64_t bits = (uint64_t)nw;
           while (bits) {
             const int b = __ffsll((unsigned long long)bits) - 1;
             bits &= bits - 1;
@@ -241,7 +263,7 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
   }
   // Lanes that never arrived: INF.
   for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
-    uint64_t miss = ~seen[v] & lanemask;
+    uint64_t miss = (uint64_t)(LT)(~seen[v] & lanemask);
     while (miss) {
       const int b = __ffsll((unsigned long long)miss) - 1;
       miss &= miss - 1;
@@ -466,8 +488,18 @@ __global__ void __launch_bounds__(BS) batch_kernel(BatchArgs a) {
     if (G.weighted)
       run_weighted<BS>(a, G, T, (uint32_t*)next, (uint32_t*)seen, qv, (int32_t*)qw, s_off, s_beg,
                        (int*)s_f, s_scan, s_cnt, s_red);
+    else if (T.lanes <= 8)
+      run_unweighted<BS, uint8_t>(a, G, T, (uint8_t*)seen, (uint8_t*)next, qv, (uint8_t*)qw,
+                                  touched, s_off, s_beg, s_f, s_scan, s_cnt);
+    else if (T.lanes <= 16)
+      run_unweighted<BS, uint16_t>(a, G, T, (uint16_t*)seen, (uint16_t*)next, qv, (uint16_t*)qw,
+                                   touched, s_off, s_beg, s_f, s_scan, s_cnt);
+    else if (T.lanes <= 32)
+      run_unweighted<BS, uint32_t>(a, G, T, (uint32_t*)seen, (uint32_t*)next, qv, (uint32_t*)qw,
+                                   touched, s_off, s_beg, s_f, s_scan, s_cnt);
     else
-      run_unweighted<BS>(a, G, T, seen, next, qv, qw, touched, s_off, s_beg, s_f, s_scan, s_cnt);
+      run_unweighted<BS, uint64_t>(a, G, T, seen, next, qv, qw, touched, s_off, s_beg, s_f,
+                                   s_scan, s_cnt);
     __syncthreads();
   }
 }

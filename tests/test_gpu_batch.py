@@ -31,6 +31,15 @@ def test_batch_small_uneven(orc, weighted):
     assert st["steps"] > 0
 
 
+@pytest.mark.parametrize("spg", [3, 12, 24, 40, 64])
+def test_batch_lane_widths(orc, spg):
+    """Tasks keep their lanes in the narrowest word that holds them (1, 2, 4
+    or 8 bytes per vertex); every width gives the oracle's rows."""
+    print(f"case: 10 uneven graphs n in [1k, 8k], {spg} sources each", flush=True)
+    graphs, src, _ = wl.uneven(10, seed=11, spg=spg, nmax=8000)
+    check_batch(orc, graphs, src, spg, False)
+
+
 def test_batch_isolated_and_duplicate_sources(orc):
     h = dawn.HostGraph.from_edges(50, np.arange(0, 20, dtype=np.int32), np.arange(1, 21, dtype=np.int32))
     h2 = dawn.HostGraph.er(300, 3.0, 3)
