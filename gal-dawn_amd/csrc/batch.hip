@@ -67,6 +67,7 @@ struct BatchArgs {
   int32_t* qv;         // [grid][2][max_n]
   uint64_t* qw;        // [grid][2][max_n]
   int32_t* touched;    // [grid][max_n]
+  uint16_t* stage;     // [grid][max_n][8]: arrival levels of tasks of <= 8 lanes
 };
 
 namespace {
@@ -127,6 +128,9 @@ __device__ __forceinline__ int lds_append(unsigned* s_count, bool pred) {
   return pred ? (int)(base + (unsigned)__popcll(m & lanes_below_mask())) : -1;
 }
 
+// Stage entry of a level that does not fit 16 bits (written to its row).
+constexpr uint16_t kStaged = 0xFFFF;
+
 // atomicOr on a lane word of type LT; sub-word types go through the
 // aligned 32-bit word that holds them.  Returns the old value of *p.
 template <typename LT>
@@ -159,6 +163,12 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
   // Lane b of the task is source lane0 + b of the graph: row drow + b * n.
   int32_t* drow = a.dist + G.dist_off + (int64_t)T.lane0 * n;
   const LT lanemask = (LT)(T.lanes >= 64 ? ~0ull : ((1ull << T.lanes) - 1));
+  // Tasks of at most 8 lanes stage each arrival level in stage[v][b]
+  // (16 bits; kStaged marks a level that did not fit, written to its row at
+  // once) and write the rows in one coalesced pass at the end, instead of
+  // 4-byte stores scattered over the rows as vertices are reached.
+  constexpr bool kStage = sizeof(LT) == 1;
+  uint16_t* stage = a.stage + (int64_t)blockIdx.x * mn * 8;
   for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
     seen[v] = (LT)~lanemask;
     next[v] = 0;
@@ -175,7 +185,8 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
         s_cnt[0]++;
       }
       seen[s] |= bit;
-      drow[(int64_t)i * n + s] = 0;
+      if constexpr (kStage) stage[(int64_t)s * 8 + i] = 0;
+      else drow[(int64_t)i * n + s] = 0;
     }
     // Words of the level-0 entries.
     for (unsigned j = 0; j < s_cnt[0]; j++) qw[j] = seen[qv[j]] & lanemask;
@@ -245,6 +256,13 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
           while (bits) {
             const int b = __ffsll((unsigned long long)bits) - 1;
             bits &= bits - 1;
+            if constexpr (kStage) {
+              if (lv < (int32_t)kStaged) {
+                stage[(int64_t)w * 8 + b] = (uint16_t)lv;
+                continue;
+              }
+              stage[(int64_t)w * 8 + b] = kStaged;
+            }
             drow[(int64_t)b * n + w] = lv;
           }
         }
@@ -261,13 +279,25 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
     level = lv;
     __syncthreads();
   }
-  // Lanes that never arrived: INF.
+  // Rows: staged levels, INF for lanes that never arrived (one thread per
+  // vertex, so each row store of a wave covers 64 consecutive vertices).
   for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
-    uint64_t miss = (uint64_t)(LT)(~seen[v] & lanemask);
-    while (miss) {
-      const int b = __ffsll((unsigned long long)miss) - 1;
-      miss &= miss - 1;
-      drow[(int64_t)b * n + v] = kInf;
+    const uint64_t got = (uint64_t)(LT)(seen[v] & lanemask);
+    if constexpr (kStage) {
+      const uint4 q = *reinterpret_cast<const uint4*>(stage + (int64_t)v * 8);
+      const uint32_t hw[4] = {q.x, q.y, q.z, q.w};
+      for (int b = 0; b < T.lanes; b++) {
+        const uint32_t l = (hw[b >> 1] >> (16 * (b & 1))) & 0xFFFFu;
+        if (!((got >> b) & 1)) drow[(int64_t)b * n + v] = kInf;
+        else if (l != kStaged) drow[(int64_t)b * n + v] = (int32_t)l;
+      }
+    } else {
+      uint64_t miss = (uint64_t)(LT)(~seen[v] & lanemask);
+      while (miss) {
+        const int b = __ffsll((unsigned long long)miss) - 1;
+        miss &= miss - 1;
+        drow[(int64_t)b * n + v] = kInf;
+      }
     }
   }
   if (threadIdx.x == 0) atomicAdd(a.levels_total, (unsigned long long)level);
@@ -557,6 +587,7 @@ extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
   DevBuf<BatchTask> dt;
   DevBuf<int32_t> ds, dq, dtouch, dout;
   DevBuf<uint64_t> dseen, dnext, dqw;
+  DevBuf<uint16_t> dstage;
   DevBuf<unsigned> dcnt;
   DevBuf<unsigned long long> dlev, dwsum;
   DAWN_TRY(dg.alloc(num_graphs));
@@ -567,6 +598,7 @@ extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
   DAWN_TRY(dq.alloc((size_t)grid * 2 * max_n));
   DAWN_TRY(dqw.alloc((size_t)grid * 2 * max_n));
   DAWN_TRY(dtouch.alloc((size_t)grid * max_n));
+  DAWN_TRY(dstage.alloc((size_t)grid * max_n * 8));
   DAWN_TRY(dcnt.alloc(1));
   DAWN_TRY(dlev.alloc(1));
   DAWN_TRY(dwsum.alloc(num_graphs));
@@ -599,6 +631,7 @@ extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
   a.qv = dq.p;
   a.qw = dqw.p;
   a.touched = dtouch.p;
+  a.stage = dstage.p;
   hipEvent_t e0, e1;
   DAWN_HIP(hipEventCreate(&e0));
   DAWN_HIP(hipEventCreate(&e1));
