@@ -40,6 +40,23 @@ def test_batch_lane_widths(orc, spg):
     check_batch(orc, graphs, src, spg, False)
 
 
+def test_batch_levels_beyond_16_bits(orc):
+    """A path of 70,000 vertices: levels past 65,534 do not fit the 16-bit
+    level stage and go to the rows directly; rows are |i - s|."""
+    n = 70000
+    print(f"case: path of {n} vertices, 3 sources, plus a small ER graph", flush=True)
+    u = np.arange(n - 1, dtype=np.int32)
+    path = dawn.HostGraph.from_edges(n, u, u + 1)
+    er = dawn.HostGraph.er(500, 3.0, 9)
+    src = np.array([0, n - 1, 20000, 1, 2, 3], np.int32)
+    check_batch(orc, [path, er], src, 3, False)
+    g = dawn.Graph(path)
+    want = np.abs(np.arange(n)[None, :] - src[:3, None]).astype(np.int32)
+    got, _ = dawn.batch_mssp([g], src[:3], 3)
+    assert np.array_equal(got[0], want) and got[0].max() == n - 1
+    g.close()
+
+
 def test_batch_isolated_and_duplicate_sources(orc):
     h = dawn.HostGraph.from_edges(50, np.arange(0, 20, dtype=np.int32), np.arange(1, 21, dtype=np.int32))
     h2 = dawn.HostGraph.er(300, 3.0, 3)
