@@ -307,11 +307,28 @@ __device__ void run_unweighted(const BatchArgs& a, const BatchGraph& G, const Ba
 // (src/algorithm/cpu/sssp.cpp:40-46): listed out-edges of the source write
 // their weight in row order, the distinct neighbours form the first
 // frontier, distance[source] = 0, the source is never relaxed.  Then
-// near-far relaxation to the same fixed point.  mark[v] = id of the last
-// near queue v was appended to; farflag[v] = v is in the far pile.
+// near-far relaxation to the same fixed point.  flags[v], one byte: kNear
+// (v is in the near queue, cleared when v is taken out to be relaxed) and
+// kFar (v is in the far pile).  A byte per vertex instead of two 32-bit
+// words keeps more of the task's randomly probed state in the caches; a
+// relaxation clears kNear and then reads dist[u] at the L2, so an
+// improvement that found kNear still set is relaxed with its new value.
+constexpr unsigned kNear = 1u, kFar = 2u;
+
+__device__ __forceinline__ unsigned flags_or(uint8_t* f, unsigned bits) {
+  const uintptr_t ad = (uintptr_t)f;
+  const int sh = (int)(ad & 3) * 8;
+  return (atomicOr((unsigned*)(ad & ~(uintptr_t)3), bits << sh) >> sh) & 0xFFu;
+}
+__device__ __forceinline__ void flags_clear(uint8_t* f, unsigned bits) {
+  const uintptr_t ad = (uintptr_t)f;
+  const int sh = (int)(ad & 3) * 8;
+  atomicAnd((unsigned*)(ad & ~(uintptr_t)3), ~(bits << sh));
+}
+
 template <int BS>
 __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const BatchTask& T,
-                             uint32_t* mark, uint32_t* farflag, int32_t* qv, int32_t* far,
+                             uint8_t* flags, int32_t* qv, int32_t* far,
                              int* s_off, int* s_beg, int* s_du, int* s_scan, unsigned* s_cnt,
                              int* s_red) {
   const int32_t n = G.n;
@@ -325,8 +342,7 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
   const long long delta = (long long)fmax(1.0, 2.0 * avg_w / fmax(avg_deg, 1.0));
   for (int32_t v = threadIdx.x; v < n; v += blockDim.x) {
     dist[v] = kInf;
-    mark[v] = 0u;
-    farflag[v] = 0u;
+    flags[v] = 0;
   }
   __syncthreads();
   // s_cnt: [0] near queue length, [1] next near length, [3] far length,
@@ -342,12 +358,12 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
     unsigned c = 0, f = 0;
     for (int32_t k = b; k < e; k++) {
       const int32_t v = G.col[k];
-      if (v == src || mark[v] == 1u || farflag[v]) continue;
+      if (v == src || flags[v]) continue;
       if (dist[v] < thr) {
-        mark[v] = 1u;
+        flags[v] = (uint8_t)kNear;
         qv[c++] = v;
       } else {
-        farflag[v] = 1u;
+        flags[v] = (uint8_t)kFar;
         far[f++] = v;
       }
     }
@@ -359,14 +375,12 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
   unsigned qn = s_cnt[0];
   long long thr = s_red[1];
   int par = 0, fpar = 0;
-  uint32_t step = 1;
   uint32_t steps = 0;
   for (;;) {
     while (qn > 0) {
       const int32_t* q_in = qv + (int64_t)par * mn;
       int32_t* q_out = qv + (int64_t)(par ^ 1) * mn;
       int32_t* f_out = far + (int64_t)fpar * mn;
-      step++;
       steps++;
       if (threadIdx.x == 0) s_cnt[1] = 0;
       __syncthreads();
@@ -377,7 +391,8 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
           const int32_t u = q_in[i];
           beg = G.row_ptr[u];
           deg = G.row_ptr[u + 1] - beg;
-          du = dist[u];
+          flags_clear(flags + u, kNear);
+          du = __hip_atomic_load(dist + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         int total;
         const int off = scan_bs<BS>(deg, s_scan, &total);
@@ -397,8 +412,8 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
             if (v != src && nd < dist[v]) {
               const int32_t old = atomicMin(&dist[v], nd);
               if (nd < old) {
-                if ((long long)nd < thr) add_near = atomicExch(&mark[v], step) != step;
-                else add_far = atomicExch(&farflag[v], 1u) == 0u;
+                if ((long long)nd < thr) add_near = !(flags_or(flags + v, kNear) & kNear);
+                else add_far = !(flags_or(flags + v, kFar) & kFar);
               }
             }
           }
@@ -436,7 +451,6 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
     const long long old_thr = thr;
     const int32_t fmin = s_red[0];
     thr = fmin == kInf ? old_thr : (long long)fmin + delta;
-    step++;
     if (threadIdx.x == 0) {
       s_cnt[1] = 0;
       s_cnt[4] = 0;
@@ -450,10 +464,10 @@ __device__ void run_weighted(const BatchArgs& a, const BatchGraph& G, const Batc
         v = f_in[i];
         const long long d = dist[v];
         if (d < old_thr) {
-          farflag[v] = 0u;  // relaxed as a near vertex already
+          flags_clear(flags + v, kFar);  // relaxed as a near vertex already
         } else if (d < thr) {
-          farflag[v] = 0u;
-          to_near = atomicExch(&mark[v], step) != step;
+          flags_clear(flags + v, kFar);
+          to_near = !(flags_or(flags + v, kNear) & kNear);
         } else {
           keep = true;
         }
@@ -516,7 +530,7 @@ __global__ void __launch_bounds__(BS) batch_kernel(BatchArgs a) {
     const BatchTask T = a.tasks[t];
     const BatchGraph G = a.graphs[T.graph];
     if (G.weighted)
-      run_weighted<BS>(a, G, T, (uint32_t*)next, (uint32_t*)seen, qv, (int32_t*)qw, s_off, s_beg,
+      run_weighted<BS>(a, G, T, (uint8_t*)next, qv, (int32_t*)qw, s_off, s_beg,
                        (int*)s_f, s_scan, s_cnt, s_red);
     else if (T.lanes <= 8)
       run_unweighted<BS, uint8_t>(a, G, T, (uint8_t*)seen, (uint8_t*)next, qv, (uint8_t*)qw,
