@@ -254,6 +254,9 @@ __global__ void __launch_bounds__(kBlock) w_step_kernel(WArgs a) {
 // over the wave, merged with atomicMin (a vertex's chunks may run in
 // different waves).  Light vertices take 16 source lanes at a time from
 // 16-byte record loads.
+// In-edges per round of a light vertex in the batched pull step.
+constexpr int kPullRound = 4;
+
 template <int L>
 __global__ void __launch_bounds__(kBlock) w_pull_kernel(WArgs a) {
   constexpr int CH = L < 16 ? L : 16;
@@ -298,17 +301,30 @@ __global__ void __launch_bounds__(kBlock) w_pull_kernel(WArgs a) {
 #pragma unroll
         for (int bl = 0; bl < CH; bl++) best[bl] = kInf;
         bool any = false;
-        for (int j = 0; j < deg; j++) {
-          const int32_t u = a.in_col[beg + j];
-          const uint64_t m = (act_in[u] >> (c * CH)) & kChunk;
-          if (!m) continue;
-          any = true;
-          const int32_t wt = a.in_w[beg + j];
-          int32_t du[CH];
-          load_chunk(a.dist + (int64_t)u * L + c * CH, du);
+        // Rounds of kPullRound in-edges: their columns and weights, then
+        // their tails' masks, then the improved tails' records, so that a
+        // lane waits on memory once per stage and round, not per edge.
+        for (int j0 = 0; j0 < deg; j0 += kPullRound) {
+          int32_t u[kPullRound], wt[kPullRound];
 #pragma unroll
-          for (int bl = 0; bl < CH; bl++)
-            if (((m >> bl) & 1ull) && du[bl] != kInf) best[bl] = min(best[bl], minplus(du[bl], wt));
+          for (int t = 0; t < kPullRound; t++) {
+            const bool ok = j0 + t < deg;
+            u[t] = ok ? a.in_col[beg + j0 + t] : -1;
+            wt[t] = ok ? a.in_w[beg + j0 + t] : 0;
+          }
+          uint64_t m[kPullRound];
+#pragma unroll
+          for (int t = 0; t < kPullRound; t++) m[t] = u[t] >= 0 ? (act_in[u[t]] >> (c * CH)) & kChunk : 0ull;
+#pragma unroll
+          for (int t = 0; t < kPullRound; t++) {
+            if (!m[t]) continue;
+            any = true;
+            int32_t du[CH];
+            load_chunk(a.dist + (int64_t)u[t] * L + c * CH, du);
+#pragma unroll
+            for (int bl = 0; bl < CH; bl++)
+              if (((m[t] >> bl) & 1ull) && du[bl] != kInf) best[bl] = min(best[bl], minplus(du[bl], wt[t]));
+          }
         }
         if (!any) continue;
         int32_t dv[CH];
