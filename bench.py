@@ -428,6 +428,8 @@ def main():
                     help="only the 1 GiB plain copy kernel (rocprofv3 traffic calibration)")
     ap.add_argument("--summarize-rocprof-c2", default=None, metavar="DIR",
                     help="print the tables of DIR/prof_c2q (rocprofv3 kernel trace of --quick-profile-c2)")
+    ap.add_argument("--quick-profile-c4", default=None, choices=("unweighted", "weighted"),
+                    help="one 512-graph uneven batch (8 sources each), twice, for rocprof runs")
     ap.add_argument("--quick-profile-c2", action="store_true",
                     help="the weighted 2048^2 grid only: one SSSP from the first source and one "
                          "16-source MSSP, each after one untimed run (for rocprofv3 runs)")
@@ -470,6 +472,21 @@ def main():
             out[name] = {"device_ms": st["device_ms"], "steps": st["steps"], "mode_steps": st["mode_steps"]}
         print(json.dumps(out), flush=True)
         g.close()
+        return
+    if args.quick_profile_c4:
+        # One uneven 512-graph batch (8 sources per graph), run twice, for
+        # the rocprof runs of profiles/batch_uneven512_traffic.md.
+        wl = importlib.import_module("gal_dawn_amd.workloads")
+        weighted = args.quick_profile_c4 == "weighted"
+        graphs, src, _ = wl.uneven(512, seed=4, weighted=weighted)
+        devs = [dawn.Graph(h) for h in graphs]
+        times = []
+        for _ in range(2):
+            _, st = dawn.batch_mssp(devs, src, 8)
+            times.append(st["device_ms"])
+        print(json.dumps({"quick_profile_c4": args.quick_profile_c4, "batch_ms": times}), flush=True)
+        for g in devs:
+            g.close()
         return
     if args.suite:
         if os.path.exists(TUNE_PATH):
