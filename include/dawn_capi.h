@@ -388,8 +388,12 @@ int dawn_step(const dawn_graph* g, const dawn_frontier* f, int32_t mode, void* s
 /* ------------------------------------------------------------ autotune --- */
 
 /* Autotuned launch knobs per graph class (no counterpart in the reference,
- * whose launch shape is fixed).  dawn_tune runs a sweep on g, applies the
- * winners to g and stores them in the in-process cache, from which graphs
+ * whose launch shape is fixed).  dawn_tune runs a sweep on g (unweighted:
+ * whole traversals of 4096 uniform pseudo-random sources under the learned
+ * schedule, or the fixed rule when no network is installed; weighted: two
+ * single-source traversals), applies the winners to g (a candidate must be
+ * 0.5% faster than the best so far) and stores them in the in-process
+ * cache, from which graphs
  * created later in the same class start; load/save persist the cache as
  * text, one line per class: "class dense_blocks heavy_degree pull_chunk
  * ss_chunk pull_variant lane_words hub_order". */
