@@ -8,8 +8,9 @@
 // no grid-wide synchronisation exists and every workgroup exits once the
 // task counter passes the last task).
 //
-// Unweighted task: all sources of one graph as lanes of a uint64 word
-// (<= 64 lanes), sparse level-synchronous BOVM:
+// Unweighted task: all sources of one graph (or a share of them on the
+// largest graphs) as lanes of one word per vertex, as narrow as the lanes
+// allow (<= 64 lanes; 8 sources: a byte), sparse level-synchronous BOVM:
 //   Q_l = [(v, lanes that reached v at level l)]
 //   expand: for (u, f) in Q_l, w in N(u): nv = f & ~seen[w]; if nv,
 //           atomicOr(next[w], nv); the first writer appends w to `touched`
