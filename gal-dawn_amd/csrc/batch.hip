@@ -76,9 +76,12 @@ namespace {
 // Threads per workgroup of the persistent kernel: a task (one graph's
 // traversal) runs inside one workgroup, so the largest graphs' tasks are
 // the critical path of a batch, and wider workgroups shorten it (measured
-// on 512 graphs of 1k-130k vertices: 1024 threads, two workgroups per CU,
-// 30 ms unweighted / 85 ms weighted; 256 threads, two or eight per CU:
-// 74 ms / 183-203 ms).
+// on 512 graphs of 1k-130k vertices: 1024 threads 30 ms unweighted / 85 ms
+// weighted; 256 threads, two or eight per CU: 74 ms / 183-203 ms).  One
+// such workgroup is resident per CU (the kernel's scalar registers allow
+// seven waves per SIMD); forcing two per CU (eight waves, scalar registers
+// spilled) made the batches slower (18.7 -> 24-27 ms unweighted, 46 -> 51 ms
+// weighted): twice the concurrent tasks' state no longer fits the caches.
 constexpr int kBatchBlock = 1024;
 
 template <int BS>
@@ -595,8 +598,8 @@ extern "C" int dawn_batch_mssp(dawn_graph* const* graphs, int32_t num_graphs,
   DAWN_HIP(hipSetDevice(device));
   int cus = 0;
   DAWN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-  // Two workgroups per CU (32 waves).
-  const int grid = std::max(1, std::min<int>((int)ht.size(), cus * 2));
+  // One workgroup per CU: as many as are resident at once.
+  const int grid = std::max(1, std::min<int>((int)ht.size(), cus));
 
   DevBuf<BatchGraph> dg;
   DevBuf<BatchTask> dt;
