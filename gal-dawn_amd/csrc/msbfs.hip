@@ -1757,8 +1757,8 @@ unsigned ms_fin_grid(const dawn_graph* g, int W) {
 }
 // Distance pass: 1024/W vertices per workgroup (W > 4: W/4 workgroups per
 // 256-vertex tile, on one XCD, so the grid is a multiple of kXcds * W/4), at
-// most 32768 workgroups (measured on RMAT-24 at W = 16: 8192 and 65536 are
-// both about 4% slower).
+// most 32768 workgroups (RMAT-24 at W = 16, 4096-source passes alternating
+// on one box: 16384 makes the pass about 1.7% slower, 65536 the same).
 unsigned ms_final_grid(const dawn_graph* g, int W) {
   unsigned grid = grid_for(std::max<int64_t>(g->n, 1), kBlock * 4 / W, 32768u);
   if (W > 4) {
